@@ -1,0 +1,127 @@
+/*
+ * lemhip.h — C-ABI of the MI355X (gfx950) landscape-evolution timestep.
+ *
+ * Drop-in boundary for the per-timestep hot path of PARALLEM's lem.cpp driver
+ * (lem.cpp:83-97).  The reference binds this path as C++ free functions
+ * taking (Data* data, Data* device[, int iter]); a context replaces the
+ * "device" Data mirror and keeps all state resident in HBM across phases and
+ * timesteps, so the reference's per-phase allocate/copy/free calls
+ * (memory_dev.cu:33-240) become no-ops.  Host buffers passed here are
+ * caller-owned, in the reference's host layout: row-major W*H cells, double
+ * fields, int fd/SFD/mask, and Slopes as 8 doubles per cell
+ * (Slopes[8*i + d], d = N, NE, E, SE, S, SW, W, NW).
+ *
+ * Conventions: every call returns 0 on success or a negative LEMHIP_E_* code
+ * (never exits the process); lemhip_last_error() describes the last failure.
+ * One context per host thread; work runs on the context's own HIP stream and
+ * every call is synchronous at the boundary.
+ */
+#ifndef LEMHIP_H
+#define LEMHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LEMHIP_OK 0
+#define LEMHIP_E_ARG (-1)
+#define LEMHIP_E_HIP (-2)
+#define LEMHIP_E_NOMEM (-3)
+#define LEMHIP_E_NOT_CONVERGED (-4)
+#define LEMHIP_E_STATE (-5)
+
+/* Precision forms. */
+#define LEMHIP_FP64 0        /* reference dtypes, bit-exact with the FP64 restatement */
+#define LEMHIP_FP32_KAHAN 1  /* FP32 bulk, FP64 dem/soilT, float-float compensated fa */
+
+/* Model parameters: the positional par file (default.par:3-35, read by
+ * io.cpp:376-580) plus the grid's MapInfo (MapInfo.h:4-12). */
+typedef struct {
+    int width, height;
+    double cellsize;
+    int flowdirectiontype;       /* 0 SFD, 1 MFD */
+    double stoneval, fineval, soilTval, soilMval, soilBval, nutval;
+    int soil_struct, profile_permeability;
+    double Qthreshold, Gslope, kappa, Gflowlim, Flow_A, Flow_B, max_incision;
+    double ddk, dck, dgk, temp_inflex, temp_sens;
+    double rain, temp;           /* runin rain (mm/yr) and temperature (C) */
+    int runiniter, start_iter;   /* correctmfdflow's first-iteration outlet rule (mfd_simple.cu:336) */
+} lem_params;
+
+/* Per-step totals the reference writes to its budgets CSV (io.cpp:233-263). */
+typedef struct {
+    double FA_max, totE, totI, totG, totD, totweatherC, totweatherP, totbio;
+    int outletcellidx;
+    int sinkcount;
+    int sinks_pass1;
+    int nlabels, nedges;
+    int plateau_rounds1, plateau_rounds2;
+    int fa_uncomputed, sed_uncomputed;
+} lem_summary;
+
+/* Fields for upload/download (host layout as described above). */
+enum lem_field {
+    LEM_F_DEM = 0, LEM_F_MASK = 1, LEM_F_SLOPES = 2, LEM_F_SLOPEPTR = 3,
+    LEM_F_SOILT = 4, LEM_F_SOILM = 5, LEM_F_STONE = 6, LEM_F_FINES = 7,
+    LEM_F_NUT = 8, LEM_F_SOILB = 9, LEM_F_TOTB = 10,
+    LEM_F_RAIN = 11, LEM_F_TEMP = 12, LEM_F_LAST_RAIN = 13, LEM_F_LAST_TEMP = 14,
+    LEM_F_FD = 15, LEM_F_SFD = 16, LEM_F_RUNOFF = 17, LEM_F_FA = 18,
+    LEM_F_ERO = 19, LEM_F_INCI = 20, LEM_F_GELI = 21, LEM_F_DEPO = 22,
+    LEM_F_DZ = 23, LEM_F_WC = 24, LEM_F_WP = 25,
+    LEM_F_COUNT = 26
+};
+
+typedef struct lemhip_ctx lemhip_ctx;
+
+/* Replaces createDeviceSpace (memory_dev.cu:293-321) and the per-phase
+ * setdevicespace_* / cleardevicespace_* calls (memory_dev.cu:33-240): allocates
+ * every device array once, on device `device_id`. */
+int lemhip_create(const lem_params *p, int precision, int device_id, lemhip_ctx **out);
+/* Replaces clearDeviceSpace (memory_dev.cu:324-347). */
+int lemhip_destroy(lemhip_ctx *ctx);
+const char *lemhip_last_error(const lemhip_ctx *ctx);
+
+/* Host state initialisation with the reference's init semantics, done on the
+ * device: createmask (memory.cu:217-242), init_setoutletcell (Data.cpp:38-66),
+ * createSoilTfromformula (memory.cu:197-214; soilT_mode 1 clamps it at 0),
+ * SetClimateMatrices(-10000) (Data.cpp:211-303), setProcessMatrices
+ * (Data.cpp:115-185), copyMask / copylastclimate (memory_dev.cu:242-289).
+ * rain may be NULL (rain gradient of SetClimateMatrices) or a per-cell mm/yr
+ * field that replaces it. */
+int lemhip_init_state(lemhip_ctx *ctx, const double *dem, const double *rain, int soilT_mode);
+
+/* Upload/download one field (host layout). */
+int lemhip_upload(lemhip_ctx *ctx, int field, const void *host);
+int lemhip_download(lemhip_ctx *ctx, int field, void *host);
+int lemhip_get_outlet(lemhip_ctx *ctx, int *outlet);
+int lemhip_set_outlet(lemhip_ctx *ctx, int outlet);
+
+/* Phases of one timestep (lem.cpp:83-97). */
+int lemhip_flow_direction(lemhip_ctx *ctx, int iter); /* cuFlowDirection, routeflow.cu:686-975 */
+int lemhip_runoff(lemhip_ctx *ctx);                   /* computeRunOffWeights + calcpropsSFD, runoffweight.cu:61-138 */
+int lemhip_flow_accum(lemhip_ctx *ctx, int iter);     /* correctmfdflow, mfd_simple.cu:238-384 */
+int lemhip_erosion(lemhip_ctx *ctx, int iter);        /* erosionGPU, erosion.cu:69-186 */
+int lemhip_step(lemhip_ctx *ctx, int iter);           /* all four, lem.cpp:83-97 */
+int lemhip_run(lemhip_ctx *ctx, int iter0, int nsteps);
+
+/* Climate forcing: SetClimateMatrices(iter) (Data.cpp:231-271) with the given
+ * scalar rain/temp, and copytolastclimate (memory_dev.cu:363-373). */
+int lemhip_set_climate(lemhip_ctx *ctx, double rain, double temp);
+int lemhip_copy_to_last_climate(lemhip_ctx *ctx);
+
+/* Totals of the last step (the Thrust reductions of erosion.cu:96-115,
+ * mfd_accum.cu:459-461, eroincidep.cu:446-455, updates.cu:218-220,
+ * mfd_simple.cu:354-355). */
+int lemhip_summary(lemhip_ctx *ctx, lem_summary *out);
+
+/* Test hook with no counterpart in the reference: evaluates the device FP64
+ * math routines (fn 0 exp, 1 log, 2 log10, 3 pow, 4 tanh, 5 expm1) on n inputs. */
+int lemhip_math_eval(int fn, int n, const double *x, const double *y, double *out);
+
+/* Device properties for measurement code (no reference counterpart). */
+int lemhip_device_info(int device_id, char *name, int name_len, int *cu_count, double *mem_gb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+// lem_ctx.h — the context that owns one DEM's device-resident state.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "lem_device.h"
+#include "../../include/lemhip.h"
+
+// per-cell status of the dependency engine
+enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2 };
+
+struct lemhip_ctx {
+    lem_params p;
+    int prec = LEMHIP_FP64;
+    int dev = 0;
+    hipStream_t s = nullptr;
+    int W = 0, H = 0;
+    long N = 0;
+    size_t tsz = 8;            // bytes of one bulk element (8 FP64, 4 FP32 form)
+    std::string err;
+
+    // ---- persistent state (HBM-resident across steps) ----
+    double *dem = nullptr;     // f64 in every form
+    double *soilT = nullptr;   // f64 in every form (sediment mass)
+    uint8_t *mask = nullptr;
+    void *slopes = nullptr;    // 8 planes of N (d-major), T
+    void *soilM = nullptr, *stone = nullptr, *fines = nullptr, *nut = nullptr, *soilB = nullptr, *TotB = nullptr;
+    void *rain = nullptr, *temp = nullptr, *last_rain = nullptr, *last_temp = nullptr;
+
+    // ---- per-step products ----
+    uint8_t *fd = nullptr, *SFD = nullptr;
+    void *w = nullptr;         // runoff weight, T
+    void *fa = nullptr;        // f64 (FP64) or float2 hi/lo (FP32 form)
+    void *slopeptr = nullptr, *ero = nullptr, *inci = nullptr, *geli = nullptr, *depo = nullptr;
+    void *dz = nullptr, *wC = nullptr, *wP = nullptr;
+
+    // ---- scratch ----
+    float *sp0 = nullptr, *sp1 = nullptr;
+    int *cand = nullptr;       // plateau candidates (also holds pass-2 list)
+    int *ws = nullptr;         // watershed labels
+    int *scan_tmp = nullptr;   // sink flags / prefix
+    double *z2 = nullptr;      // raised dem (transient)
+    void *cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    unsigned long long *hkeys = nullptr, *hvals = nullptr;  // edge hash table
+    long hcap = 0;
+    uint32_t *elo = nullptr, *ehi = nullptr;
+    unsigned long long *ew = nullptr;                        // compacted edges (weight encoded)
+    long ecap = 0;
+    unsigned long long *rmm = nullptr;                       // minimax per label
+    uint32_t *comp = nullptr;                                // component of labels cut off from label 0
+    unsigned long long *cmax = nullptr;
+    uint8_t *seen = nullptr;
+    long lcap = 0;
+    uint8_t *status = nullptr;  // DAG engine status
+    uint8_t *dmask = nullptr;   // DAG donor bits (pull order)
+    int *tile_flags = nullptr;  // per tile: 1 = finished
+    double *partials = nullptr; // deterministic reductions
+    LemScalars *scal = nullptr;   // device
+    LemScalars *hscal = nullptr;  // pinned host mirror
+    void *stage = nullptr;        // pinned host staging for upload/download
+    size_t stage_bytes = 0;
+    int *hint = nullptr;          // pinned host ints for small readbacks
+
+    // counters known on the host after each phase
+    int h_rounds1 = 0, h_rounds2 = 0, h_nlab = 0, h_nedges = 0, h_sinkcount = 0, h_sinks1 = 0;
+    int h_fa_unc = 0, h_sed_unc = 0;
+
+    // tiling of the dependency engine
+    int tw = 64, th = 16;
+    int ntx = 0, nty = 0;
+};
+
+#define LEM_CHECK(ctx, call)                                                              \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return LEMHIP_E_HIP;                                                          \
+        }                                                                                 \
+    } while (0)
+
+#define LEM_LAUNCHED(ctx)                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess) {                                                           \
+            (ctx)->err = std::string("kernel launch: ") + hipGetErrorString(e_);          \
+            return LEMHIP_E_HIP;                                                          \
+        }                                                                                 \
+    } while (0)
+
+#define LEM_TRY(x)                  \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != LEMHIP_OK) return r_; \
+    } while (0)
+
+// phase entry points implemented in the .hip files
+int lem_fd_phase(lemhip_ctx *c, int iter);
+int lem_runoff_phase(lemhip_ctx *c);
+int lem_fa_phase(lemhip_ctx *c, int iter);
+int lem_mod_phase(lemhip_ctx *c, int iter);
+int lem_sync_scalars(lemhip_ctx *c);
+int lem_sum_fields(lemhip_ctx *c, const void *const *fields, const int *is_f64, int nf, double *out_dev);

@@ -1,0 +1,57 @@
+// lem_device.h — device-side constants and helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/lem_math.h"
+
+// Direction index d = 0..7 runs N, NE, E, SE, S, SW, W, NW (routeflow.cu:701-702);
+// ESRI codes per direction (routeflow.h:32).
+__device__ __constant__ static const int8_t kDX[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+__device__ __constant__ static const int8_t kDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+__device__ __constant__ static const uint8_t kCODE[8] = {64, 128, 1, 2, 4, 8, 16, 32};
+// pull order of CalcMFD / MFDsedbud: E, SE, S, SW, W, NW, N, NE (mfd_simple.cu:95-165)
+__device__ __constant__ static const int8_t kPULL[8] = {2, 3, 4, 5, 6, 7, 0, 1};
+
+#define LEM_NODATA (-9999.0)
+
+// code a neighbour in direction d must carry to flow into the centre cell
+__device__ __forceinline__ uint8_t donor_code(int d) { return kCODE[(d + 4) & 7]; }
+
+// direction index of an ESRI code (0 if not a single code)
+__device__ __forceinline__ int code_dir(int code)
+{
+    switch (code) {
+    case 64: return 0; case 128: return 1; case 1: return 2; case 2: return 3;
+    case 4: return 4; case 8: return 5; case 16: return 6; default: return 7;
+    }
+}
+
+// order-preserving map double -> uint64 (for atomic min/max on doubles)
+__device__ __host__ __forceinline__ uint64_t ord_enc(double x)
+{
+    uint64_t u = lm_to_bits(x);
+    return (u & 0x8000000000000000ULL) ? ~u : (u | 0x8000000000000000ULL);
+}
+__device__ __host__ __forceinline__ double ord_dec(uint64_t u)
+{
+    return lm_from_bits((u & 0x8000000000000000ULL) ? (u & 0x7FFFFFFFFFFFFFFFULL) : ~u);
+}
+
+// per-step scalars kept on the device
+struct LemScalars {
+    double lowest;        // min over mask cells of dem, starting at 1000 (routeflow.cu:171-184)
+    double base_level;    // dem[outlet] - 0.00012 (updates.cu:151)
+    double FA_max;
+    double totE, totI, totG, totD, totwC, totwP, totbio;
+    int outlet;
+    int sinks_pass1;      // mask cells with SFD == 0 after pass-1 routing
+    int sinkcount;        // distinct labels in the edge set (newflow.cu:1045) or sinks_pass1
+    int nlabels;
+    int nedges;
+    int plateau_rounds1, plateau_rounds2;
+    int fa_uncomputed, sed_uncomputed;
+    int ncand;            // plateau candidates (mask cells without a downhill neighbour)
+    int flag;             // generic change flag for host-driven loops
+    int hash_overflow;
+    unsigned long long lowest_enc;
+};

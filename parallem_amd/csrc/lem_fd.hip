@@ -1,0 +1,747 @@
+// lem_fd.hip — flow direction phase (cuFlowDirection, routeflow.cu:686-975):
+// D8/MFD directions, plateau routing, depression (sink) routing by watershed
+// labels and minimax spill heights, the second direction pass on the raised
+// surface, and the fd/SFD merge.
+#include "lem_ctx.h"
+#include <hipcub/hipcub.hpp>
+
+// ------------------------------------------------------------------ helpers
+
+// one atomic per wavefront: returns this lane's slot, or -1 if !pred
+__device__ __forceinline__ int wave_append(int *counter, bool pred)
+{
+    unsigned long long b = __ballot(pred);
+    if (b == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)b) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(b));
+    base = __shfl(base, leader);
+    return pred ? base + __popcll(b & ((1ULL << lane) - 1ULL)) : -1;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_reduce_sum(T v, T *sh)
+{
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------- A5: lowest
+
+// min over mask cells of dem with the reference's start value 1000 and strict
+// '<' (NaN never wins), as an order-preserving atomic min (exact, order-free)
+__global__ void k_lowest(long N, const double *__restrict__ z, const uint8_t *__restrict__ mask,
+                         unsigned long long *out)
+{
+    __shared__ unsigned long long sh[256];
+    unsigned long long best = ord_enc(1000.0);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double v = z[i];
+        if (mask[i] && v < 1000.0) {
+            unsigned long long e = ord_enc(v);
+            if (e < best) best = e;
+        }
+    }
+    sh[threadIdx.x] = best;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s && sh[threadIdx.x + s] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(out, sh[0]);
+}
+
+// routeflow.cu:739-741: the outlet is lowered below the lowest cell after the
+// directions were computed on the unadjusted surface
+__global__ void k_outlet_adjust(double *dem, LemScalars *sc)
+{
+    double lowest = ord_dec(sc->lowest_enc);
+    sc->lowest = lowest;
+    if (lowest < dem[sc->outlet]) dem[sc->outlet] = lowest - 0.0001;
+}
+
+// ------------------------------------------------------------ A2: FlowDirs
+
+// newflow.cu:49-174.  One thread per cell; a wave covers 64 consecutive cells
+// of a row so the centre row and both neighbour rows are read coalesced.
+// Slopes are written only for downhill neighbours (sticky state, D7); pass 2
+// passes slopes == nullptr.  Cells without a downhill neighbour are appended
+// to the plateau candidate list (the cells whose initial distance is 2e7).
+template <typename TS>
+__global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, const double *__restrict__ z,
+                                                  const uint8_t *__restrict__ mask, TS *__restrict__ slopes,
+                                                  long plane, uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
+                                                  int *__restrict__ cand, int *__restrict__ ncand)
+{
+    int icol = blockIdx.x * blockDim.x + threadIdx.x;
+    int irow = blockIdx.y * blockDim.y + threadIdx.y;
+    bool inside = icol < W && irow < H;
+    long self = (long)irow * W + icol;
+    bool active = inside && mask[self];
+    int aspect = 0, aspectsfd = 0;
+    if (active) {
+        double zc = z[self];
+        double smax = 0.0;
+        int up = 0, flat = 0;
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            int cx = icol + kDX[d], cy = irow + kDY[d];
+            if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+                float dc = (d & 1) ? 1.41f : 1.0f;
+                float den = (float)cell_size * dc;
+                double zn = z[(long)cy * W + cx];
+                double stemp = (zc - zn) / (double)den;
+                if (zn == -9999) stemp = -0.0000001;
+                if ((zc > zn) && (zn != -9999)) {
+                    if (slopes) slopes[d * plane + self] = (TS)stemp;
+                    if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
+                    aspect += kCODE[d];
+                }
+                if (zc < zn) up++;
+                if (zc == zn) flat++;
+            }
+        }
+        if (up > 7) { aspect = 0; aspectsfd = 0; }
+        if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
+        if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
+    }
+    if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; }
+    int slot = wave_append(ncand, active && aspect == 0 && aspectsfd == 0);
+    if (slot >= 0) cand[slot] = (int)self;
+}
+
+// ------------------------------------------------------ A3/A4: plateaus
+
+__global__ void k_sp_init(long N, float *sp0, float *sp1)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        sp0[i] = 0.0f;
+        sp1[i] = 0.0f;
+    }
+}
+
+__global__ void k_sp_seed(const int *cand, int n, float *sp0, float *sp1)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) { sp0[cand[k]] = 20000000.0f; sp1[cand[k]] = 20000000.0f; }
+}
+
+// route_plateausMFD (newflow.cu:264-350) as one synchronous round (D1): every
+// read sees the previous round's distances; each strict improvement in the
+// scan marks that slope and takes its direction.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_plateau_round(int W, int H, const int *__restrict__ cand, int n,
+                                                       const double *__restrict__ z, const float *__restrict__ sp_prev,
+                                                       float *__restrict__ sp_next, uint8_t *__restrict__ fd,
+                                                       uint8_t *__restrict__ SFD, TS *__restrict__ slopes, long plane,
+                                                       int *flag)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    long self = cand[k];
+    int irow = (int)(self / W), icol = (int)(self % W);
+    float min_distance = sp_prev[self];
+    if (min_distance == 0) { sp_next[self] = min_distance; return; }
+    double zc = z[self];
+    bool changed = false;
+    int code = -1;
+    for (int d = 0; d < 8; d++) {
+        int cx = icol + kDX[d], cy = irow + kDY[d];
+        if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+            float dc = (d & 1) ? 1.41f : 1.0f;
+            long nb = (long)cy * W + cx;
+            float dis = sp_prev[nb] + dc;
+            if (zc == z[nb] && min_distance > dis) {
+                min_distance = dis;
+                changed = true;
+                code = kCODE[d];
+                if (slopes) slopes[d * plane + self] = (TS)0.0001;
+            }
+        }
+    }
+    sp_next[self] = min_distance;
+    if (changed) {
+        fd[self] = (uint8_t)code;
+        SFD[self] = (uint8_t)code;
+        *flag = 1;
+    }
+}
+
+__global__ void k_count_sinks(const int *cand, int n, const uint8_t *SFD, int *out)
+{
+    __shared__ int sh[256];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = (k < n && SFD[cand[k]] == 0) ? 1 : 0;
+    v = block_reduce_sum<int>(v, sh);
+    if (threadIdx.x == 0 && v) atomicAdd(out, v);
+}
+
+// ------------------------------------------ A7-A9: watershed labels
+
+// sink flag: mask, not on the border rows/last column, SFD == 0 (newflow.cu:363-378)
+__global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *SFD, int *flag)
+{
+    long N = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        int r = (int)(i / W), c = (int)(i % W);
+        bool border = (c == W - 1) || (r == 0) || (r == H - 1);
+        flag[i] = (mask[i] && !border && SFD[i] == 0) ? 1 : 0;
+    }
+}
+
+// init_watershedMFD with labels = 1 + row-major rank of the sink (D3)
+__global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD, const int *prefix, int *ws)
+{
+    long N = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        int r = (int)(i / W), c = (int)(i % W);
+        int v;
+        if (!mask[i]) v = 0;
+        else if ((c == W - 1) || (r == 0) || (r == H - 1)) v = 0;
+        else if (SFD[i] == 0) v = prefix[i] + 1;
+        else {
+            int d = code_dir(SFD[i]);
+            v = -((r + kDY[d]) * W + (c + kDX[d]) + 1);
+        }
+        ws[i] = v;
+    }
+}
+
+// init_watershed_sinkMFD (newflow.cu:430-460): SFD==0 cells adopt the largest
+// label of an equal-height neighbour; iterated to the fixed point (the
+// component maximum, independent of the update order).  SFD==0 cells are a
+// subset of the pass-1 plateau candidates.
+__global__ void k_sink_merge(int W, int H, const int *cand, int n, const uint8_t *SFD, const double *z,
+                             int *ws, int *flag)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    long self = cand[k];
+    if (SFD[self] != 0) return;
+    int irow = (int)(self / W), icol = (int)(self % W);
+    double zc = z[self];
+    int cur = ws[self], best = cur;
+    for (int d = 0; d < 8; d++) {
+        int cx = icol + kDX[d], cy = irow + kDY[d];
+        if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+            long nb = (long)cy * W + cx;
+            if (z[nb] == zc) {
+                int v = ws[nb];
+                if (v > best) best = v;
+            }
+        }
+    }
+    if (best > cur) { ws[self] = best; *flag = 1; }
+}
+
+// identify_watershedMFD (newflow.cu:469-487): pointer jumping; in-place reads
+// of already-jumped targets only shorten chains
+__global__ void k_jump(long N, const uint8_t *mask, int *ws, int *flag)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        int v = ws[i];
+        if (v < 0 && mask[i]) {
+            int t = ws[-v - 1];
+            ws[i] = t;
+            if (t < 0) *flag = 1;
+        }
+    }
+}
+
+// ---------------------------------------------- A10-A11: edges, spill
+
+#define HASH_EMPTY 0xFFFFFFFFFFFFFFFFULL
+
+__global__ void k_hash_clear(long cap, unsigned long long *keys, unsigned long long *vals)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < cap; i += (long)gridDim.x * blockDim.x) {
+        keys[i] = HASH_EMPTY;
+        vals[i] = HASH_EMPTY;
+    }
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k)
+{
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+    return k;
+}
+
+// identify_watershed_boundary_write_backMFD (newflow.cu:787-877) without the
+// fixed table (D4): every mask cell c and in-grid neighbour n with different
+// labels and z_c >= z_n contributes edge {ws_c, ws_n} at height z_c; the
+// table keeps the minimum per unordered pair (order-free, exact).
+__global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__restrict__ mask, const int *__restrict__ ws,
+                                               const double *__restrict__ z, unsigned long long *keys,
+                                               unsigned long long *vals, long cap, int *overflow)
+{
+    int icol = blockIdx.x * blockDim.x + threadIdx.x;
+    int irow = blockIdx.y * blockDim.y + threadIdx.y;
+    if (icol >= W || irow >= H) return;
+    long c = (long)irow * W + icol;
+    if (!mask[c]) return;
+    int me = ws[c];
+    double zc = z[c];
+    int seen[8];
+    int ns = 0;
+    for (int d = 0; d < 8; d++) {
+        int cx = icol + kDX[d], cy = irow + kDY[d];
+        if (cx < 0 || cx >= W || cy < 0 || cy >= H) continue;
+        long nb = (long)cy * W + cx;
+        int other = ws[nb];
+        if (other == me) continue;
+        if (!(zc >= z[nb])) continue;
+        bool dup = false;
+        for (int q = 0; q < ns; q++) dup |= (seen[q] == other);
+        if (dup) continue;
+        seen[ns++] = other;
+        unsigned lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
+        unsigned long long key = ((unsigned long long)lo << 32) | hi;
+        unsigned long long wenc = ord_enc(zc);
+        unsigned long long m = (unsigned long long)(cap - 1);
+        unsigned long long h = mix64(key) & m;
+        long probes = 0;
+        while (true) {
+            unsigned long long prev = atomicCAS(&keys[h], HASH_EMPTY, key);
+            if (prev == HASH_EMPTY || prev == key) { atomicMin(&vals[h], wenc); break; }
+            h = (h + 1) & m;
+            if (++probes >= cap) { *overflow = 1; break; }
+        }
+    }
+}
+
+__global__ void k_edges_compact(long cap, const unsigned long long *keys, const unsigned long long *vals,
+                                uint32_t *elo, uint32_t *ehi, unsigned long long *ew, int *count, uint8_t *seen)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < cap; i += (long)gridDim.x * blockDim.x) {
+        unsigned long long k = keys[i];
+        bool have = k != HASH_EMPTY;
+        int slot = wave_append(count, have);
+        if (have) {
+            uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)(k & 0xffffffffu);
+            elo[slot] = lo;
+            ehi[slot] = hi;
+            ew[slot] = vals[i];
+            seen[lo] = 1;
+            seen[hi] = 1;
+        }
+    }
+}
+
+__global__ void k_count_bytes(long n, const uint8_t *a, int *out)
+{
+    __shared__ int sh[256];
+    int acc = 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) acc += a[i];
+    acc = block_reduce_sum<int>(acc, sh);
+    if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
+}
+
+__global__ void k_minimax_init(long n, unsigned long long *r, uint8_t *seen)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        r[i] = (i == 0) ? ord_enc(-lm_inf()) : ord_enc(lm_inf());
+        seen[i] = 0;
+    }
+}
+
+// Bottleneck distance to label 0 over the basin graph:
+// r[b] = min over edges (b,c) of max(w_bc, r[c]), r[0] = -inf.  This is the
+// value the reference's Kruskal/union-find pass assigns (routeflow.cu:529-572):
+// a basin is raised to the height at which it first joins label 0 in
+// ascending-weight order.  Relaxation in any order reaches the same fixed point.
+__global__ void k_minimax_relax(int ne, const uint32_t *elo, const uint32_t *ehi, const unsigned long long *ew,
+                                unsigned long long *r, int *flag)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    uint32_t a = elo[k], b = ehi[k];
+    unsigned long long w = ew[k];
+    unsigned long long ra = r[a], rb = r[b];
+    unsigned long long ca = w > rb ? w : rb;
+    unsigned long long cb = w > ra ? w : ra;
+    if (ca < ra) { atomicMin(&r[a], ca); *flag = 1; }
+    if (cb < rb) { atomicMin(&r[b], cb); *flag = 1; }
+}
+
+// Basins with no path to label 0 (only possible when NaN heights cut them
+// off): the reference's union-find never marks them done, merges them in
+// ascending order and leaves every member with the height of the component's
+// last merge, i.e. the largest edge of the component's minimum spanning tree.
+// That equals max over v of the bottleneck distance from any one member, so
+// the component's smallest label seeds a second relaxation and the component
+// takes the maximum.
+__global__ void k_fb_unreached(int ne, const uint32_t *elo, const uint32_t *ehi, const unsigned long long *r, int *flag)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ne && (r[elo[k]] == INF || r[ehi[k]] == INF)) *flag = 1;
+}
+
+__global__ void k_fb_comp_init(long n, uint32_t *comp, unsigned long long *cmax)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        comp[i] = (uint32_t)i;
+        cmax[i] = 0ULL;
+    }
+}
+
+__global__ void k_fb_comp_relax(int ne, const uint32_t *elo, const uint32_t *ehi, const unsigned long long *r,
+                                uint32_t *comp, int *flag)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    uint32_t a = elo[k], b = ehi[k];
+    if (r[a] != INF || r[b] != INF) return;
+    uint32_t ca = comp[a], cb = comp[b];
+    if (ca < cb) { atomicMin(&comp[b], ca); *flag = 1; }
+    else if (cb < ca) { atomicMin(&comp[a], cb); *flag = 1; }
+}
+
+// seeds: the smallest label of each cut-off component; other cut-off labels
+// restart from +inf in a separate array (cmax doubles as that array)
+__global__ void k_fb_seed(long n, const unsigned long long *r, const uint32_t *comp, const uint8_t *seen,
+                          unsigned long long *r2)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        r2[i] = (r[i] == INF && seen[i] && comp[i] == (uint32_t)i) ? ord_enc(-lm_inf()) : INF;
+}
+
+__global__ void k_fb_max(long n, const unsigned long long *r, const uint32_t *comp, const uint8_t *seen,
+                         const unsigned long long *r2, unsigned long long *out)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        if (r[i] == INF && seen[i] && r2[i] != INF && r2[i] != ord_enc(-lm_inf())) atomicMax(&out[comp[i]], r2[i]);
+}
+
+__global__ void k_fb_apply(long n, unsigned long long *r, const uint32_t *comp, const uint8_t *seen,
+                           const unsigned long long *out)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        if (r[i] == INF && seen[i] && out[comp[i]] != 0ULL) r[i] = out[comp[i]];
+}
+
+// raise_watershedMFD (newflow.cu:879-894) into a copy of the surface; label 0
+// and labels without a path keep raise = 0 as in the reference's zeroed array
+__global__ void k_raise(long N, const uint8_t *mask, const int *ws, const unsigned long long *r,
+                        const double *z, double *z2)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double v = z[i];
+        if (mask[i]) {
+            int lab = ws[i];
+            unsigned long long e = r[lab];
+            double rs = (lab == 0 || e == INF) ? 0.0 : ord_dec(e);
+            if (v < rs) v = rs;
+        }
+        z2[i] = v;
+    }
+}
+
+// routeflow.cu:938-953
+__global__ void k_merge(long N, int flowtype, const uint8_t *mask, uint8_t *fd, const uint8_t *SFD, const LemScalars *sc)
+{
+    int outlet = sc->outlet;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        uint8_t f = fd[i];
+        if (flowtype == 0) f = SFD[i];
+        else if (f == 0 && mask[i]) f = SFD[i];
+        if (i == outlet) f = 16;
+        fd[i] = f;
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+static int grid1d(long n, int block = 256, int cap = 8192)
+{
+    long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+static int read_scalars(lemhip_ctx *c)
+{
+    LEM_CHECK(c, hipMemcpyAsync(c->hscal, c->scal, sizeof(LemScalars), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    return LEMHIP_OK;
+}
+
+static int clear_flag(lemhip_ctx *c)
+{
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->flag, 0, sizeof(int), c->s));
+    return LEMHIP_OK;
+}
+
+static int read_flag(lemhip_ctx *c, int *v)
+{
+    LEM_CHECK(c, hipMemcpyAsync(&c->hscal->flag, &c->scal->flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    *v = c->hscal->flag;
+    return LEMHIP_OK;
+}
+
+// FlowDirs + plateau routing on surface zs; slopes == nullptr for pass 2.
+// Leaves the candidate list in c->cand and its length in *ncand_out; returns
+// the number of rounds that changed something in *rounds.
+template <typename TS>
+static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, int *ncand_out, int *rounds)
+{
+    long N = c->N;
+    int W = c->W, H = c->H;
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
+    dim3 blk(64, 4);
+    dim3 grd((W + 63) / 64, (H + 3) / 4);
+    k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, N, c->fd, c->SFD,
+                                          c->cand, &c->scal->ncand);
+    LEM_LAUNCHED(c);
+    LEM_CHECK(c, hipMemcpyAsync(&c->hscal->ncand, &c->scal->ncand, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    int n = c->hscal->ncand;
+    *ncand_out = n;
+    *rounds = 0;
+    if (n == 0) return LEMHIP_OK;
+    k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, c->sp0, c->sp1);
+    k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(c->cand, n, c->sp0, c->sp1);
+    LEM_LAUNCHED(c);
+    float *a = c->sp0, *b = c->sp1;
+    for (;;) {
+        LEM_TRY(clear_flag(c));
+        k_plateau_round<TS><<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, n, zs, a, b, c->fd, c->SFD,
+                                                                      slopes, N, &c->scal->flag);
+        LEM_LAUNCHED(c);
+        int f = 0;
+        LEM_TRY(read_flag(c, &f));
+        float *t = a; a = b; b = t;
+        if (!f) break;
+        ++*rounds;
+        if (*rounds > n + 2) { c->err = "plateau routing did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    return LEMHIP_OK;
+}
+
+static int ensure_hash(lemhip_ctx *c, long want)
+{
+    long cap = 1024;
+    while (cap < want) cap <<= 1;
+    if (cap <= c->hcap) return LEMHIP_OK;
+    if (c->hkeys) { (void)hipFree(c->hkeys); (void)hipFree(c->hvals); (void)hipFree(c->elo); (void)hipFree(c->ehi); (void)hipFree(c->ew); }
+    c->hkeys = c->hvals = c->ew = nullptr;
+    c->elo = c->ehi = nullptr;
+    c->hcap = 0;
+    LEM_CHECK(c, hipMalloc(&c->hkeys, cap * 8));
+    LEM_CHECK(c, hipMalloc(&c->hvals, cap * 8));
+    LEM_CHECK(c, hipMalloc(&c->elo, cap * 4));
+    LEM_CHECK(c, hipMalloc(&c->ehi, cap * 4));
+    LEM_CHECK(c, hipMalloc(&c->ew, cap * 8));
+    c->hcap = cap;
+    c->ecap = cap;
+    return LEMHIP_OK;
+}
+
+static int ensure_labels(lemhip_ctx *c, long nl)
+{
+    if (nl <= c->lcap) return LEMHIP_OK;
+    if (c->rmm) { (void)hipFree(c->rmm); (void)hipFree(c->seen); (void)hipFree(c->comp); (void)hipFree(c->cmax); }
+    c->rmm = nullptr;
+    c->seen = nullptr;
+    c->comp = nullptr;
+    c->cmax = nullptr;
+    long cap = nl + nl / 4 + 1024;
+    LEM_CHECK(c, hipMalloc(&c->rmm, cap * 8));
+    LEM_CHECK(c, hipMalloc(&c->seen, cap));
+    LEM_CHECK(c, hipMalloc(&c->comp, cap * 4));
+    LEM_CHECK(c, hipMalloc(&c->cmax, cap * 8));
+    c->lcap = cap;
+    return LEMHIP_OK;
+}
+
+static int cutoff_components(lemhip_ctx *c, int nlab, int ne)
+{
+    long n = (long)nlab + 1;
+    // second relaxation array: reuse the hash value table (>= 8*(nlab+1) slots)
+    unsigned long long *r2 = c->hvals;
+    k_fb_comp_init<<<grid1d(n), 256, 0, c->s>>>(n, c->comp, c->cmax);
+    LEM_LAUNCHED(c);
+    for (int it = 0;; ++it) {
+        LEM_TRY(clear_flag(c));
+        k_fb_comp_relax<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->rmm, c->comp, &c->scal->flag);
+        LEM_LAUNCHED(c);
+        int f = 0;
+        LEM_TRY(read_flag(c, &f));
+        if (!f) break;
+        if (it > nlab + 2) { c->err = "component labelling did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    k_fb_seed<<<grid1d(n), 256, 0, c->s>>>(n, c->rmm, c->comp, c->seen, r2);
+    LEM_LAUNCHED(c);
+    for (int it = 0;; ++it) {
+        LEM_TRY(clear_flag(c));
+        k_minimax_relax<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->ew, r2, &c->scal->flag);
+        LEM_LAUNCHED(c);
+        int f = 0;
+        LEM_TRY(read_flag(c, &f));
+        if (!f) break;
+        if (it > nlab + 2) { c->err = "cut-off spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    k_fb_max<<<grid1d(n), 256, 0, c->s>>>(n, c->rmm, c->comp, c->seen, r2, c->cmax);
+    k_fb_apply<<<grid1d(n), 256, 0, c->s>>>(n, c->rmm, c->comp, c->seen, c->cmax);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+// floodingDriver (routeflow.cu:202-676) up to the raised surface in c->z2
+static int flooding(lemhip_ctx *c, int ncand1)
+{
+    long N = c->N;
+    int W = c->W, H = c->H;
+    // labels
+    k_sink_flags<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, c->scan_tmp);
+    LEM_LAUNCHED(c);
+    size_t need = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->scan_tmp, c->ws, (int)N, c->s);
+    if (need > c->cub_bytes) {
+        if (c->cub_tmp) (void)hipFree(c->cub_tmp);
+        c->cub_tmp = nullptr;
+        LEM_CHECK(c, hipMalloc(&c->cub_tmp, need));
+        c->cub_bytes = need;
+    }
+    // prefix lands in ws, then k_init_ws rewrites ws in place from it: use z2 as the prefix buffer instead
+    int *prefix = reinterpret_cast<int *>(c->z2);
+    LEM_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, need, c->scan_tmp, prefix, (int)N, c->s));
+    LEM_CHECK(c, hipMemcpyAsync(&c->hint[0], c->scan_tmp + (N - 1), sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipMemcpyAsync(&c->hint[1], prefix + (N - 1), sizeof(int), hipMemcpyDeviceToHost, c->s));
+    k_init_ws<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, prefix, c->ws);
+    LEM_LAUNCHED(c);
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    int nlab = c->hint[0] + c->hint[1];
+    c->h_nlab = nlab;
+    for (int it = 0;; ++it) {
+        LEM_TRY(clear_flag(c));
+        k_sink_merge<<<grid1d(ncand1, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, ncand1, c->SFD, c->dem, c->ws,
+                                                                     &c->scal->flag);
+        LEM_LAUNCHED(c);
+        int f = 0;
+        LEM_TRY(read_flag(c, &f));
+        if (!f) break;
+        if (it > ncand1 + 2) { c->err = "sink merge did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    for (int it = 0;; ++it) {
+        LEM_TRY(clear_flag(c));
+        k_jump<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, &c->scal->flag);
+        LEM_LAUNCHED(c);
+        int f = 0;
+        LEM_TRY(read_flag(c, &f));
+        if (!f) break;
+        if (it > 64) { c->err = "watershed pointer jumping did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    // edges
+    LEM_TRY(ensure_labels(c, (long)nlab + 1));
+    long want = 8L * (nlab + 1);
+    if (want > 4 * N) want = 4 * N;
+    if (want < 1024) want = 1024;
+    for (int attempt = 0;; ++attempt) {
+        LEM_TRY(ensure_hash(c, want * 2));
+        k_hash_clear<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals);
+        LEM_CHECK(c, hipMemsetAsync(&c->scal->hash_overflow, 0, sizeof(int), c->s));
+        dim3 blk(64, 4), grd((W + 63) / 64, (H + 3) / 4);
+        k_edges<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->hkeys, c->hvals, c->hcap,
+                                       &c->scal->hash_overflow);
+        LEM_LAUNCHED(c);
+        LEM_CHECK(c, hipMemcpyAsync(&c->hscal->hash_overflow, &c->scal->hash_overflow, sizeof(int),
+                                    hipMemcpyDeviceToHost, c->s));
+        LEM_CHECK(c, hipStreamSynchronize(c->s));
+        if (!c->hscal->hash_overflow) break;
+        want = c->hcap * 2;
+        if (attempt > 8) { c->err = "edge table overflow"; return LEMHIP_E_NOMEM; }
+    }
+    k_minimax_init<<<grid1d(nlab + 1), 256, 0, c->s>>>(nlab + 1, c->rmm, c->seen);
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->sinkcount, 0, sizeof(int), c->s));
+    k_edges_compact<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals, c->elo, c->ehi, c->ew,
+                                                       &c->scal->nedges, c->seen);
+    k_count_bytes<<<grid1d(nlab + 1, 256, 1024), 256, 0, c->s>>>(nlab + 1, c->seen, &c->scal->sinkcount);
+    LEM_LAUNCHED(c);
+    LEM_TRY(read_scalars(c));
+    int ne = c->hscal->nedges;
+    c->h_nedges = ne;
+    c->h_sinkcount = c->hscal->sinkcount;
+    for (int it = 0;; ++it) {
+        LEM_TRY(clear_flag(c));
+        if (ne > 0)
+            k_minimax_relax<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->ew, c->rmm,
+                                                                        &c->scal->flag);
+        LEM_LAUNCHED(c);
+        int f = 0;
+        LEM_TRY(read_flag(c, &f));
+        if (!f) break;
+        if (it > nlab + 2) { c->err = "spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    LEM_TRY(clear_flag(c));
+    if (ne > 0) k_fb_unreached<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->rmm, &c->scal->flag);
+    LEM_LAUNCHED(c);
+    int cut = 0;
+    LEM_TRY(read_flag(c, &cut));
+    if (cut) LEM_TRY(cutoff_components(c, nlab, ne));
+    k_raise<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, c->rmm, c->dem, c->z2);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+template <typename TS>
+static int fd_phase_t(lemhip_ctx *c, int iter)
+{
+    (void)iter;
+    long N = c->N;
+    // lowest cell for the outlet check, taken on the unadjusted surface
+    unsigned long long init = ord_enc(1000.0);
+    LEM_CHECK(c, hipMemcpyAsync(&c->scal->lowest_enc, &init, 8, hipMemcpyHostToDevice, c->s));
+    k_lowest<<<grid1d(N, 256, 2048), 256, 0, c->s>>>(N, c->dem, c->mask, &c->scal->lowest_enc);
+    LEM_LAUNCHED(c);
+    // pass 1 on the unraised surface; slopes persist (D7)
+    int n1 = 0, r1 = 0;
+    LEM_TRY(directions_and_plateaus<TS>(c, c->dem, reinterpret_cast<TS *>(c->slopes), &n1, &r1));
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->sinks_pass1, 0, sizeof(int), c->s));
+    if (n1 > 0) {
+        k_count_sinks<<<grid1d(n1, 256, 1 << 30), 256, 0, c->s>>>(c->cand, n1, c->SFD, &c->scal->sinks_pass1);
+        LEM_LAUNCHED(c);
+    }
+    LEM_CHECK(c, hipMemcpyAsync(&c->hscal->sinks_pass1, &c->scal->sinks_pass1, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    int sinks = c->hscal->sinks_pass1;
+    c->h_sinks1 = sinks;
+    c->h_rounds1 = r1;
+    c->h_rounds2 = 0;
+    c->h_nlab = 0;
+    c->h_nedges = 0;
+    c->h_sinkcount = sinks;
+    if (sinks > 0) {
+        LEM_TRY(flooding(c, n1));
+        int n2 = 0, r2 = 0;
+        LEM_TRY(directions_and_plateaus<TS>(c, c->z2, (TS *)nullptr, &n2, &r2));
+        c->h_rounds2 = r2;
+    }
+    k_merge<<<grid1d(N), 256, 0, c->s>>>(N, c->p.flowdirectiontype, c->mask, c->fd, c->SFD, c->scal);
+    k_outlet_adjust<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+int lem_fd_phase(lemhip_ctx *c, int iter)
+{
+    if (c->prec == LEMHIP_FP64) return fd_phase_t<double>(c, iter);
+    return fd_phase_t<float>(c, iter);
+}

@@ -1,0 +1,439 @@
+// lem_mod.hip — runoff weights (runoffweight.cu:21-138), the elementwise
+// erosion/weathering/update kernels (eroincidep.cu, updates.cu) fused per
+// cell in the reference's order, the sediment pass (lem_dag.hip) between
+// them, and deterministic reductions for the per-step totals.
+#include "lem_ctx.h"
+
+template <typename T> int lem_fa_dag(lemhip_ctx *c);
+template <typename T> int lem_sed_dag(lemhip_ctx *c);
+
+// ---------------------------------------------------------------- math
+
+// FP64: the shared portable routines (bit-identical with the host build);
+// FP32 form: the device library's single-precision functions.
+__device__ __forceinline__ double mexp(double x) { return lm_exp(x); }
+__device__ __forceinline__ double mlog(double x) { return lm_log(x); }
+__device__ __forceinline__ double mlog10(double x) { return lm_log10(x); }
+__device__ __forceinline__ double mpow(double x, double y) { return lm_pow(x, y); }
+__device__ __forceinline__ double mtanh(double x) { return lm_tanh(x); }
+__device__ __forceinline__ float mexp(float x) { return expf(x); }
+__device__ __forceinline__ float mlog(float x) { return logf(x); }
+__device__ __forceinline__ float mlog10(float x) { return log10f(x); }
+__device__ __forceinline__ float mpow(float x, float y) { return y == 2.0f ? x * x : powf(x, y); }
+__device__ __forceinline__ float mtanh(float x) { return tanhf(x); }
+
+template <typename T> struct FaS;
+template <> struct FaS<double> { using type = double; };
+template <> struct FaS<float> { using type = float2; };
+__device__ __forceinline__ double fa_val(double v) { return v; }
+__device__ __forceinline__ float fa_val(float2 v) { return v.x; }
+__device__ __forceinline__ void fa_set(double *p, double v) { *p = v; }
+__device__ __forceinline__ void fa_set(float2 *p, float v) { *p = make_float2(v, 0.f); }
+
+static int grid1d(long n, int block = 256, int cap = 8192)
+{
+    long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// ---------------------------------------------------------------- runoff
+
+// calcrunoffweight (runoffweight.cu:21-58); also seeds fa = w on mask cells,
+// the value a cell keeps if it never completes (mfd_simple.cu:72)
+template <typename T>
+__global__ void k_runoff(long N, const uint8_t *__restrict__ mask, const T *__restrict__ rain,
+                         const T *__restrict__ stone, const T *__restrict__ TotB, T *__restrict__ soilM,
+                         T *__restrict__ w, typename FaS<T>::type *__restrict__ fa)
+{
+    for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
+        if (mask[self] == 0) {
+            w[self] = (T)-9999;
+            fa_set(&fa[self], (T)0);
+            continue;
+        }
+        T runoff_coeff = ((T)1.0 - (T)0.0085 * (stone[self])) * mexp((T)-0.0025 * (TotB[self]) / (T)76.5)
+                         * mtanh((T)0.6 + (soilM[self]) / (T)0.2);
+        T myppt = rain[self] / (T)1000;
+        T wv = myppt * runoff_coeff;
+        w[self] = wv;
+        T sm = soilM[self];
+        sm += myppt * ((T)1. - runoff_coeff);
+        if (sm > (T)0.5) sm = (T)0.5;
+        soilM[self] = sm;
+        fa_set(&fa[self], wv);
+    }
+}
+
+// ------------------------------------------------ MOD part 1 (pre-sediment)
+
+// aveslope (eroincidep.cu:8-37), diff_erosion (:39-87), conc_erosion
+// (:89-152), gelifluction (:154-198); depo = 1e-5 seed (mfd_accum.cu:390-394)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mod1(long N, lem_params p, const uint8_t *__restrict__ mask,
+                                              const T *__restrict__ slopes, const typename FaS<T>::type *__restrict__ fa,
+                                              const T *__restrict__ fines, const T *__restrict__ nut,
+                                              const T *__restrict__ stone, const T *__restrict__ TotB,
+                                              const double *__restrict__ soilT, const T *__restrict__ temp,
+                                              T *__restrict__ slopeptr, T *__restrict__ ero, T *__restrict__ inci,
+                                              T *__restrict__ geli, T *__restrict__ depo)
+{
+    const T rescale_c = (T)(1 / (10000 / (p.cellsize * p.cellsize)));
+    for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
+        if (mask[self] == 0) {
+            slopeptr[self] = 0; ero[self] = 0; inci[self] = 0; geli[self] = 0; depo[self] = 0;
+            continue;
+        }
+        T total = 0;
+        int nonzero = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            T sj = slopes[j * N + self];
+            total += sj;
+            if (sj > 0) nonzero++;
+        }
+        T sp = 0;
+        if ((total > 0) && (nonzero > 0)) sp = total / (T)nonzero;
+        slopeptr[self] = sp;
+        T sT = (T)soilT[self];
+        T fav = (T)fa_val(fa[self]);
+        T ex = mexp((T)-0.07 * (TotB[self]) / (T)76.5);
+        // diffuse
+        T erodibility = ((T)2.77e-6 * mpow(fines[self], (T)1.14) * ((T)12. - nut[self]))
+                        + ((T)0.043 * ((T)p.soil_struct - (T)2.)) + ((T)0.033 * ((T)p.profile_permeability - (T)3.));
+        if (erodibility < (T)0.0) erodibility = 0;
+        T OF = ((fav) * (T)1000) / (T)25;
+        if (OF > (T)p.Qthreshold) OF = (T)p.Qthreshold;
+        T slope = sp;
+        if (slope == (T)0.0) slope = (T)0.0001;
+        if (slope > (T)5.0) slope = (T)5.0;
+        T answer = ((erodibility * OF * OF * mpow(slope, (T)1.67) *
+                     (mpow(((T)110. - (stone[self])), (T)0.877) / (T)61.7023))) * ex;
+        if (answer > sT) answer = sT;
+        ero[self] = answer;
+        // concentrated
+        T cslope = sp;
+        if (cslope > (T)0.5) cslope = (T)0.5;
+        T max_incision = (T)p.max_incision;
+        T f_temp = ((fav) * rescale_c) - (T)p.Qthreshold;
+        T iv;
+        if (f_temp < (T)0.0) { f_temp = 0; iv = 0; }
+        else {
+            if (cslope == (T)0.0) iv = (T)p.Flow_A * mpow(f_temp, (T)p.Flow_B) * (T)0.01 * ex;
+            else iv = (T)p.Flow_A * mpow(f_temp, (T)p.Flow_B) * (cslope) * ex;
+        }
+        if (iv > max_incision) iv = max_incision;
+        if (iv > sT) {
+            T softrockinc = sT;
+            T bedrockinc = ((T)1.0 - (softrockinc / iv)) * (T)0.0001 * mpow(f_temp, (T)p.Flow_B) * (cslope) * ex;
+            iv = softrockinc + bedrockinc;
+            if (iv > max_incision) iv = max_incision;
+        }
+        inci[self] = iv;
+        // gelifluction
+        T kappa = (T)p.kappa;
+        T tc = temp[self];
+        if (tc >= (T)-20. && tc <= (T)8.87)
+            kappa = (T)0.15711 - (T)0.011835 * tc - (T)0.001195 * tc * tc + (T)0.000038 * tc * tc * tc
+                    + (T)0.000002 * tc * tc * tc * tc;
+        T scaler = (T)1 - (fav / (T)p.Gflowlim);
+        if (fav > (T)p.Gflowlim) scaler = (T)0.0;
+        T slopethreshold = sp;
+        if (slopethreshold > (T)p.Gslope) slopethreshold = (T)p.Gslope;
+        T g = kappa * slopethreshold * scaler;
+        if ((answer + g) > sT) g = sT - answer;
+        if (g < (T)0.0) g = 0;
+        geli[self] = g;
+        depo[self] = (T)0.00001;
+    }
+}
+
+// the exact sub-expression order of the reference's diff_erosion with the
+// FP64 routines: (((k*OF)*OF)*pow(s,1.67))*(pow(110-stone,0.877)/61.7023)*exp
+// is what the template above evaluates for T = double, with ex computed once
+// (exp(-0.07*TotB/76.5) is the same value at each of its call sites).
+
+// ------------------------------------------------ MOD part 2 (post-sediment)
+
+// surface_change (eroincidep.cu:200-217), weathering type 1 (:220-327),
+// Update_Surface (updates.cu:23-46), update_nutrient_content (:48-76) and
+// grow_vegetation (:78-139): each reads only its own cell, so one pass keeps
+// the reference's per-cell order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_t *__restrict__ mask,
+                                              const typename FaS<T>::type *__restrict__ fa,
+                                              const T *__restrict__ ero, const T *__restrict__ inci,
+                                              const T *__restrict__ geli, const T *__restrict__ depo,
+                                              const T *__restrict__ temp, const T *__restrict__ rain,
+                                              const T *__restrict__ last_temp, double *__restrict__ soilT,
+                                              T *__restrict__ stone, T *__restrict__ fines, T *__restrict__ soilM,
+                                              T *__restrict__ soilB, T *__restrict__ nut, T *__restrict__ TotB,
+                                              double *__restrict__ dem, T *__restrict__ dzo, T *__restrict__ wCo,
+                                              T *__restrict__ wPo, const LemScalars *__restrict__ sc)
+{
+    const double rescale_w = 10000 * (10000 / (p.cellsize * p.cellsize));
+    const double rescale_v = (10000 / (p.cellsize * p.cellsize));
+    const double base_level = sc->base_level;
+    for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
+        T tc = temp[self];
+        if (mask[self] == 0) {
+            dzo[self] = 0; wCo[self] = 0; wPo[self] = 0; TotB[self] = 0;
+            continue;
+        }
+        T dz = (depo[self] - ero[self] - inci[self] - geli[self]);
+        dzo[self] = dz;
+        // weathering
+        T rn = rain[self];
+        T PE_d = (T)1.44 * mpow(tc + (T)6., (T)2);
+        T P0_d = (T)7.7e-5, P1_d = (T)2.3;
+        T t_abs = tc + (T)273.16;
+        T t2 = t_abs * t_abs;
+        T ltemp = mlog10(t_abs);
+        T DHA = (T)-0.4883 + (T)8.074e-4 * tc;
+        T DHB = (T)-0.3241 + (T)1.600e-4 * tc;
+        T sqI = (T)0.31622776601683794; // sqrt(0.1), correctly rounded
+        T denom1 = ((T)-4. * DHA * sqI) / ((T)1. + (T)5.e-8 * DHB * sqI);
+        T gammCa_d = mpow((T)10., denom1);
+        T gammHCO3_d = mpow((T)10., (-DHA * sqI) / ((T)1. + (T)5.4e-8 * DHB * sqI));
+        T K1_d = mpow((T)10., (T)-356.3094 - (T)0.06091964 * t_abs + (T)21834.37 / t_abs + (T)126.8339 * ltemp - (T)1684915 / t2);
+        T K2_d = mpow((T)10., (T)-107.8871 - (T)0.03252849 * t_abs + (T)5151.79 / t_abs + (T)38.92561 * ltemp - (T)563713.9 / t2);
+        T KC_d = mpow((T)10., (T)-171.9065 - (T)0.077993 * t_abs + (T)2839.319 / t_abs + (T)71.595 * ltemp);
+        T KH_d = mpow((T)10., (T)108.3865 + (T)0.01985076 * t_abs - (T)6919.53 / t_abs - (T)40.45154 * ltemp + (T)669365.0 / t2);
+        T surface_flow = (T)fa_val(fa[self]) * (T)1000;
+        T evap_trans;
+        if (rn + surface_flow <= PE_d) evap_trans = rn + surface_flow;
+        else evap_trans = rn + surface_flow - PE_d;
+        double sT = soilT[self];
+        T PCO2;
+        if (sT > 0.1) PCO2 = mpow((T)10., (T)-2.5 + ((tc - (T)15.) / (T)10.));
+        else PCO2 = mpow((T)10., (T)-3.5 + ((tc - (T)15.) / (T)10.));
+        T Ca2eq = mpow(PCO2 * ((K1_d * KC_d * KH_d) / ((T)4. * K2_d * gammCa_d * gammHCO3_d * gammHCO3_d)), ((T)1. / (T)3.));
+        T wC = (T)0.0148148148 * Ca2eq * (surface_flow / (T)rescale_w);
+        if (wC <= (T)0.0) wC = 0;
+        if (wC > (T)0.001) wC = (T)0.001;
+        T wP = P0_d * mexp(-(P1_d) * (T)sT);
+        if (sizeof(T) == 8) wP = (T)(7.7e-5 * lm_exp(-(2.3) * sT));
+        wCo[self] = wC;
+        wPo[self] = wP;
+        sT = sT + (double)wC * 0.02 + (double)wP;
+        T st = stone[self], fi = fines[self], sm = soilM[self];
+        if (sT <= 0.0) {
+            sT = 0.0; st = 0; fi = 0; sm = (T)0.02;
+        } else {
+            T ratio = (T)((double)ero[self] / sT);
+            if (wP > (T)0) st += st * (mtanh(ratio) * -mtanh(wC / wP));
+            else st += st * mtanh(ratio);
+            if (st < (T)0.01) st = (T)0.01;
+            else if (st > (T)99.9) st = (T)99.9;
+            fi = (T)100. - st;
+            sm -= (T)((double)((T)1.e-3 * evap_trans) / sT);
+            if (sm < (T)0.02) sm = (T)0.02;
+            else if (sm > (T)0.5) sm = (T)0.5;
+        }
+        soilT[self] = sT;
+        stone[self] = st;
+        fines[self] = fi;
+        soilM[self] = sm;
+        // update surface (FP64 in every form)
+        double z = dem[self];
+        z = z + (double)dz - (double)wC - (double)wP;
+        if (z <= base_level) z = base_level + 0.01;
+        dem[self] = z;
+        // nutrients
+        T tb = TotB[self], sb = soilB[self], nu = nut[self];
+        if (sT <= 0.0) nu = 0;
+        T leaf_fall = tb / ((T)72.8 * mlog(tb + (T)1.4));
+        T decomposition = sb * mexp((T)0.12 * tc - (T)5.75);
+        sb = sb + leaf_fall - decomposition;
+        if (sT == 0.0 && sb > (T)0.0) nu = (T)100.0;
+        else if (sT == 0.0) nu = 0;
+        else nu = (T)(((double)sb / (1300. * sT)) * 100.);
+        if (nu > (T)100) nu = (T)100.;
+        soilB[self] = sb;
+        nut[self] = nu;
+        // vegetation
+        T delta_temp = tc - last_temp[self];
+        T tempeffect = (T)100 + ((T)100 / (tc + (T)15));
+        T total_cc = ((T)3000.0 + ((T)5.0 * (rn - tempeffect))) / (T)rescale_v;
+        if (delta_temp > (T)0) {
+            T r_plus = delta_temp * ((T)1.0 + mtanh((tc - (T)p.temp_inflex) / (T)p.temp_sens));
+            tb += ((r_plus * tb) * ((T)1. - (tb / total_cc)));
+        }
+        if (delta_temp < (T)0) {
+            T r_minus = delta_temp * ((T)1.0 - mtanh((tc - (T)p.temp_inflex) / (T)p.temp_sens));
+            tb += ((r_minus * tb) * (tb / total_cc));
+        }
+        if (delta_temp == (T)0) tb = total_cc;
+        if (tb < (T)1) tb = (T)1.0;
+        TotB[self] = tb;
+    }
+}
+
+// updates.cu:151 and :165-174: base level from the outlet before the update,
+// then the outlet is pinned 0.00024 below its pre-update value
+__global__ void k_outlet_pre(const double *dem, LemScalars *sc) { sc->base_level = dem[sc->outlet] - 0.00012; }
+__global__ void k_outlet_post(double *dem, const LemScalars *sc)
+{
+    dem[sc->outlet] = sc->base_level;
+    dem[sc->outlet] = dem[sc->outlet] - 0.00012;
+}
+
+// ------------------------------------------------ deterministic reductions
+
+#define RED_BLOCKS 512
+
+template <typename T>
+__global__ void k_sum_partial(long N, const T *__restrict__ a, double *__restrict__ partial)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+        acc += (double)a[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_final(int nb, const double *__restrict__ partial, double *out)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+template <typename T>
+static int dsum(lemhip_ctx *c, const void *a, double *out_dev)
+{
+    k_sum_partial<T><<<RED_BLOCKS, 256, 0, c->s>>>(c->N, reinterpret_cast<const T *>(a), c->partials);
+    k_sum_final<<<1, 256, 0, c->s>>>(RED_BLOCKS, c->partials, out_dev);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+__device__ __forceinline__ double fa_full(double v) { return v; }
+__device__ __forceinline__ double fa_full(float2 v) { return (double)v.x + (double)v.y; }
+
+template <typename FS>
+__global__ void k_fa_max_full(long N, const FS *__restrict__ fa, unsigned long long *out)
+{
+    __shared__ unsigned long long sh[256];
+    unsigned long long best = ord_enc(0.0);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double v = fa_full(fa[i]);
+        if (v > ord_dec(best)) best = ord_enc(v);
+    }
+    sh[threadIdx.x] = best;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s && sh[threadIdx.x + s] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, sh[0]);
+}
+
+// first cell holding the maximum (mfd_simple.cu:339-349: strict '>' scan from 0)
+template <typename FS>
+__global__ void k_fa_argfirst(long N, const FS *__restrict__ fa, const unsigned long long *mx, int *idx)
+{
+    double m = ord_dec(*mx);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+        if (m > 0.0 && fa_full(fa[i]) == m) atomicMin(idx, (int)i);
+}
+
+__global__ void k_set_famax(LemScalars *sc, const unsigned long long *mx, const int *idx, int move_outlet)
+{
+    double m = ord_dec(*mx);
+    sc->FA_max = m;
+    if (move_outlet) sc->outlet = (m > 0.0) ? *idx : 0;
+}
+
+// ---------------------------------------------------------------- phases
+
+template <typename T>
+static int runoff_t(lemhip_ctx *c)
+{
+    using FS = typename FaS<T>::type;
+    k_runoff<T><<<grid1d(c->N), 256, 0, c->s>>>(c->N, c->mask, (const T *)c->rain, (const T *)c->stone,
+                                                 (const T *)c->TotB, (T *)c->soilM, (T *)c->w, (FS *)c->fa);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+int lem_runoff_phase(lemhip_ctx *c)
+{
+    return c->prec == LEMHIP_FP64 ? runoff_t<double>(c) : runoff_t<float>(c);
+}
+
+template <typename T>
+static int fa_t(lemhip_ctx *c, int iter)
+{
+    using FS = typename FaS<T>::type;
+    LEM_TRY(lem_fa_dag<T>(c));
+    // FA_max and the first-iteration outlet rule (mfd_simple.cu:332-357)
+    unsigned long long *mx = reinterpret_cast<unsigned long long *>(c->partials + RED_BLOCKS);
+    int *idx = reinterpret_cast<int *>(c->partials + RED_BLOCKS + 1);
+    unsigned long long z0 = ord_enc(0.0);
+    int big = 0x7FFFFFFF;
+    c->hint[4] = 0;
+    LEM_CHECK(c, hipMemcpyAsync(mx, &z0, 8, hipMemcpyHostToDevice, c->s));
+    LEM_CHECK(c, hipMemcpyAsync(idx, &big, 4, hipMemcpyHostToDevice, c->s));
+    k_fa_max_full<FS><<<RED_BLOCKS, 256, 0, c->s>>>(c->N, (const FS *)c->fa, mx);
+    int first = (iter == c->p.start_iter - c->p.runiniter);
+    if (first) k_fa_argfirst<FS><<<grid1d(c->N), 256, 0, c->s>>>(c->N, (const FS *)c->fa, mx, idx);
+    k_set_famax<<<1, 1, 0, c->s>>>(c->scal, mx, idx, first);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+int lem_fa_phase(lemhip_ctx *c, int iter)
+{
+    return c->prec == LEMHIP_FP64 ? fa_t<double>(c, iter) : fa_t<float>(c, iter);
+}
+
+template <typename T>
+static int mod_t(lemhip_ctx *c, int iter)
+{
+    (void)iter;
+    using FS = typename FaS<T>::type;
+    long N = c->N;
+    int g = grid1d(N, 256, 16384);
+    k_mod1<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa, (const T *)c->fines,
+                                   (const T *)c->nut, (const T *)c->stone, (const T *)c->TotB, c->soilT,
+                                   (const T *)c->temp, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci, (T *)c->geli,
+                                   (T *)c->depo);
+    LEM_LAUNCHED(c);
+    LEM_TRY(dsum<T>(c, c->ero, &c->scal->totE));
+    LEM_TRY(dsum<T>(c, c->inci, &c->scal->totI));
+    LEM_TRY(dsum<T>(c, c->geli, &c->scal->totG));
+    LEM_TRY(lem_sed_dag<T>(c));
+    LEM_TRY(dsum<T>(c, c->depo, &c->scal->totD));
+    k_outlet_pre<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    k_mod2<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const FS *)c->fa, (const T *)c->ero, (const T *)c->inci,
+                                   (const T *)c->geli, (const T *)c->depo, (const T *)c->temp, (const T *)c->rain,
+                                   (const T *)c->last_temp, c->soilT, (T *)c->stone, (T *)c->fines, (T *)c->soilM,
+                                   (T *)c->soilB, (T *)c->nut, (T *)c->TotB, c->dem, (T *)c->dz, (T *)c->wC,
+                                   (T *)c->wP, c->scal);
+    k_outlet_post<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    LEM_LAUNCHED(c);
+    LEM_TRY(dsum<T>(c, c->wC, &c->scal->totwC));
+    LEM_TRY(dsum<T>(c, c->wP, &c->scal->totwP));
+    LEM_TRY(dsum<T>(c, c->TotB, &c->scal->totbio));
+    return LEMHIP_OK;
+}
+
+int lem_mod_phase(lemhip_ctx *c, int iter)
+{
+    return c->prec == LEMHIP_FP64 ? mod_t<double>(c, iter) : mod_t<float>(c, iter);
+}

@@ -1,0 +1,267 @@
+"""Python front end of liblemhip.so (the C-ABI in include/lemhip.h).
+
+Mirrors the reference driver's per-step calls (lem.cpp:83-97) on a context
+that keeps every field resident on the GPU:
+
+    cuFlowDirection(ctx, i)      routeflow.cu:686   -> lemhip_flow_direction
+    computeRunOffWeights(ctx)    runoffweight.cu:61 -> lemhip_runoff
+    correctmfdflow(ctx, i)       mfd_simple.cu:238  -> lemhip_flow_accum
+    erosionGPU(ctx, i)           erosion.cu:69      -> lemhip_erosion
+
+There is no CPU fallback: importing works without a GPU (so the library can
+be inspected), but creating a context needs the HIP library and a device, and
+any failure raises LemError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblemhip.so")
+
+FP64 = 0
+FP32_KAHAN = 1
+
+FIELDS = {
+    "dem": 0, "mask": 1, "Slopes": 2, "SlopePtr": 3, "soilT": 4, "soilM": 5, "stone": 6, "fines": 7,
+    "nut": 8, "soilB": 9, "TotB": 10, "rain": 11, "temp": 12, "last_rain": 13, "last_temp": 14,
+    "fd": 15, "SFD": 16, "runoff": 17, "fa": 18, "ero": 19, "inci": 20, "geli": 21, "depo": 22,
+    "dz": 23, "wC": 24, "wP": 25,
+}
+INT_FIELDS = {"mask", "fd", "SFD"}
+
+EXPORTS = [
+    "lemhip_create", "lemhip_destroy", "lemhip_last_error", "lemhip_init_state", "lemhip_upload",
+    "lemhip_download", "lemhip_get_outlet", "lemhip_set_outlet", "lemhip_flow_direction", "lemhip_runoff",
+    "lemhip_flow_accum", "lemhip_erosion", "lemhip_step", "lemhip_run", "lemhip_set_climate",
+    "lemhip_copy_to_last_climate", "lemhip_summary", "lemhip_math_eval", "lemhip_device_info",
+]
+
+
+class LemError(RuntimeError):
+    pass
+
+
+class LemParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("cellsize", C.c_double), ("flowdirectiontype", C.c_int),
+        ("stoneval", C.c_double), ("fineval", C.c_double), ("soilTval", C.c_double),
+        ("soilMval", C.c_double), ("soilBval", C.c_double), ("nutval", C.c_double),
+        ("soil_struct", C.c_int), ("profile_permeability", C.c_int),
+        ("Qthreshold", C.c_double), ("Gslope", C.c_double), ("kappa", C.c_double),
+        ("Gflowlim", C.c_double), ("Flow_A", C.c_double), ("Flow_B", C.c_double),
+        ("max_incision", C.c_double), ("ddk", C.c_double), ("dck", C.c_double), ("dgk", C.c_double),
+        ("temp_inflex", C.c_double), ("temp_sens", C.c_double),
+        ("rain", C.c_double), ("temp", C.c_double),
+        ("runiniter", C.c_int), ("start_iter", C.c_int),
+    ]
+
+
+class LemSummary(C.Structure):
+    _fields_ = [
+        ("FA_max", C.c_double), ("totE", C.c_double), ("totI", C.c_double), ("totG", C.c_double),
+        ("totD", C.c_double), ("totweatherC", C.c_double), ("totweatherP", C.c_double),
+        ("totbio", C.c_double),
+        ("outletcellidx", C.c_int), ("sinkcount", C.c_int), ("sinks_pass1", C.c_int),
+        ("nlabels", C.c_int), ("nedges", C.c_int), ("plateau_rounds1", C.c_int),
+        ("plateau_rounds2", C.c_int), ("fa_uncomputed", C.c_int), ("sed_uncomputed", C.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def default_params(width: int, height: int, flowtype: int = 0, cellsize: float = 30.0,
+                   runiniter: int = 10) -> LemParams:
+    """default.par:3-35 on a width x height grid of `cellsize` m cells."""
+    return LemParams(width=width, height=height, cellsize=cellsize, flowdirectiontype=flowtype,
+                     stoneval=20.0, fineval=80.0, soilTval=1.0, soilMval=0.5, soilBval=0.0, nutval=10.0,
+                     soil_struct=2, profile_permeability=3, Qthreshold=6.0, Gslope=1.0, kappa=0.00001,
+                     Gflowlim=600.0, Flow_A=0.00001, Flow_B=1.3, max_incision=0.001, ddk=0.9, dck=0.4,
+                     dgk=0.9, temp_inflex=6.0, temp_sens=6.0, rain=600.0, temp=10.0,
+                     runiniter=runiniter, start_iter=-1000)
+
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Load liblemhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LemError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       "(make -C parallem_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.lemhip_create.argtypes = [C.POINTER(LemParams), C.c_int, C.c_int, C.POINTER(vp)]
+    L.lemhip_destroy.argtypes = [vp]
+    L.lemhip_last_error.argtypes = [vp]
+    L.lemhip_last_error.restype = C.c_char_p
+    L.lemhip_init_state.argtypes = [vp, vp, vp, C.c_int]
+    L.lemhip_upload.argtypes = [vp, C.c_int, vp]
+    L.lemhip_download.argtypes = [vp, C.c_int, vp]
+    L.lemhip_get_outlet.argtypes = [vp, C.POINTER(C.c_int)]
+    L.lemhip_set_outlet.argtypes = [vp, C.c_int]
+    for f in ("lemhip_flow_direction", "lemhip_flow_accum", "lemhip_erosion", "lemhip_step"):
+        getattr(L, f).argtypes = [vp, C.c_int]
+    L.lemhip_runoff.argtypes = [vp]
+    L.lemhip_run.argtypes = [vp, C.c_int, C.c_int]
+    L.lemhip_set_climate.argtypes = [vp, C.c_double, C.c_double]
+    L.lemhip_copy_to_last_climate.argtypes = [vp]
+    L.lemhip_summary.argtypes = [vp, C.POINTER(LemSummary)]
+    L.lemhip_math_eval.argtypes = [C.c_int, C.c_int, vp, vp, vp]
+    L.lemhip_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class LemContext:
+    """One DEM with its state resident on the GPU."""
+
+    def __init__(self, params: LemParams, precision: int = FP64, device: int = 0):
+        self.L = lib()
+        self.params = params
+        self.precision = precision
+        self.W, self.H = params.width, params.height
+        self.N = self.W * self.H
+        h = C.c_void_p()
+        rc = self.L.lemhip_create(C.byref(params), precision, device, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.lemhip_last_error(h).decode() if h.value else "lemhip_create failed"
+            if h.value:
+                self.L.lemhip_destroy(h)
+            self.h = C.c_void_p()
+            raise LemError(f"lemhip_create: {rc} {msg}")
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise LemError(f"{what}: {rc} {self.L.lemhip_last_error(self.h).decode()}")
+
+    def close(self) -> None:
+        if self.h and self.h.value:
+            self.L.lemhip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- state -----------------------------------------------------------
+    def init_state(self, dem: np.ndarray, rain: np.ndarray | None = None, soilT_mode: str = "formula") -> None:
+        z = np.ascontiguousarray(dem, np.float64).reshape(-1)
+        assert z.size == self.N
+        r = None if rain is None else np.ascontiguousarray(rain, np.float64).reshape(-1)
+        mode = {"formula": 0, "nonneg": 1}[soilT_mode]
+        self._check(self.L.lemhip_init_state(self.h, _ptr(z), None if r is None else _ptr(r), mode), "init_state")
+
+    def download(self, name: str) -> np.ndarray:
+        f = FIELDS[name]
+        if name == "Slopes":
+            out = np.empty(8 * self.N, np.float64)
+        elif name in INT_FIELDS:
+            out = np.empty(self.N, np.int32)
+        else:
+            out = np.empty(self.N, np.float64)
+        self._check(self.L.lemhip_download(self.h, f, _ptr(out)), f"download {name}")
+        return out
+
+    def upload(self, name: str, a: np.ndarray) -> None:
+        f = FIELDS[name]
+        dt = np.int32 if name in INT_FIELDS else np.float64
+        a = np.ascontiguousarray(a, dt).reshape(-1)
+        self._check(self.L.lemhip_upload(self.h, f, _ptr(a)), f"upload {name}")
+
+    @property
+    def outlet(self) -> int:
+        o = C.c_int()
+        self._check(self.L.lemhip_get_outlet(self.h, C.byref(o)), "get_outlet")
+        return o.value
+
+    @outlet.setter
+    def outlet(self, v: int) -> None:
+        self._check(self.L.lemhip_set_outlet(self.h, int(v)), "set_outlet")
+
+    # --- phases (lem.cpp:83-97) -----------------------------------------
+    def flow_direction(self, it: int) -> None:
+        self._check(self.L.lemhip_flow_direction(self.h, it), "flow_direction")
+
+    def runoff(self) -> None:
+        self._check(self.L.lemhip_runoff(self.h), "runoff")
+
+    def flow_accum(self, it: int) -> None:
+        self._check(self.L.lemhip_flow_accum(self.h, it), "flow_accum")
+
+    def erosion(self, it: int) -> None:
+        self._check(self.L.lemhip_erosion(self.h, it), "erosion")
+
+    def step(self, it: int) -> None:
+        self._check(self.L.lemhip_step(self.h, it), "step")
+
+    def run(self, it0: int, n: int) -> None:
+        self._check(self.L.lemhip_run(self.h, it0, n), "run")
+
+    def set_climate(self, rain: float, temp: float) -> None:
+        self._check(self.L.lemhip_set_climate(self.h, rain, temp), "set_climate")
+
+    def copy_to_last_climate(self) -> None:
+        self._check(self.L.lemhip_copy_to_last_climate(self.h), "copy_to_last_climate")
+
+    def summary(self) -> dict:
+        s = LemSummary()
+        self._check(self.L.lemhip_summary(self.h, C.byref(s)), "summary")
+        return s.as_dict()
+
+
+# reference-named entry points (lem.cpp:84-96)
+def cuFlowDirection(ctx: LemContext, iteration: int) -> None:
+    ctx.flow_direction(iteration)
+
+
+def computeRunOffWeights(ctx: LemContext) -> None:
+    ctx.runoff()
+
+
+def correctmfdflow(ctx: LemContext, iteration: int) -> None:
+    ctx.flow_accum(iteration)
+
+
+def erosionGPU(ctx: LemContext, iteration: int) -> None:
+    ctx.erosion(iteration)
+
+
+def math_eval(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(x if y is None else y, np.float64)
+    out = np.empty_like(x)
+    rc = lib().lemhip_math_eval(fn, x.size, _ptr(x), _ptr(y), _ptr(out))
+    if rc != 0:
+        raise LemError(f"lemhip_math_eval: {rc}")
+    return out
+
+
+def device_info(dev: int = 0) -> dict:
+    name = C.create_string_buffer(64)
+    cu = C.c_int()
+    mem = C.c_double()
+    rc = lib().lemhip_device_info(dev, name, 64, C.byref(cu), C.byref(mem))
+    if rc != 0:
+        raise LemError("lemhip_device_info failed")
+    return {"arch": name.value.decode(), "cus": cu.value, "mem_gb": mem.value}
