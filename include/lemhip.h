@@ -114,6 +114,17 @@ int lemhip_copy_to_last_climate(lemhip_ctx *ctx);
  * mfd_simple.cu:354-355). */
 int lemhip_summary(lemhip_ctx *ctx, lem_summary *out);
 
+/* Live timing with HIP events on the context stream (no reference
+ * counterpart; the reference logs phase times from cudaEvents at
+ * routeflow.cu:955-971, mfd_simple.cu:376-380, mfd_accum.cu:474-480,
+ * erosion.cu:171-181).  Slots: 0 FD, 1 FlowDirs kernel, 2 plateau rounds,
+ * 3 depression routing, 4 runoff kernel, 5 flow accumulation, 6 MOD part 1
+ * kernel, 7 sediment routing, 8 MOD part 2 kernel, 9 reductions, 10 step.
+ * enable resets the accumulators; read returns milliseconds and launch
+ * counts per slot. */
+int lemhip_profile_enable(lemhip_ctx *ctx, int on);
+int lemhip_profile_read(lemhip_ctx *ctx, int n, double *ms, long long *launches);
+
 /* Test hook with no counterpart in the reference: evaluates the device FP64
  * math routines (fn 0 exp, 1 log, 2 log10, 3 pow, 4 tanh, 5 expm1) on n inputs. */
 int lemhip_math_eval(int fn, int n, const double *x, const double *y, double *out);

@@ -134,13 +134,19 @@ static void free_all(lemhip_ctx *c)
     void *ptrs[] = {c->dem, c->soilT, c->mask, c->slopes, c->soilM, c->stone, c->fines, c->nut, c->soilB, c->TotB,
                     c->rain, c->temp, c->last_rain, c->last_temp, c->fd, c->SFD, c->w, c->fa, c->slopeptr, c->ero,
                     c->inci, c->geli, c->depo, c->dz, c->wC, c->wP, c->sp0, c->sp1, c->cand, c->ws, c->scan_tmp,
-                    c->z2, c->cub_tmp, c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
-                    c->dmask, c->tile_flags, c->partials, c->scal};
+                    c->z2, c->pl_front0, c->pl_front1, c->pl_U, c->pl_stamp, c->pl_Uv, c->pl_ctl, c->cub_tmp, c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
+                    c->dmask, c->tile_flags, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake, c->dag_ctl, c->partials, c->scal};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->hint) (void)hipHostFree(c->hint);
+    if (c->prof_ready)
+        for (int k = 0; k < PROF_COUNT; k++)
+            for (int q = 0; q < PROF_MAXP; q++) {
+                (void)hipEventDestroy(c->prof[k].a[q]);
+                (void)hipEventDestroy(c->prof[k].b[q]);
+            }
     if (c->s) (void)hipStreamDestroy(c->s);
 }
 
@@ -183,6 +189,13 @@ extern "C" int lemhip_create(const lem_params *p, int precision, int device_id, 
     LEM_TRY(A(&c->fa, N * 8));  // double, or float2 hi/lo
     LEM_TRY(A((void **)&c->sp0, N * 4));
     LEM_TRY(A((void **)&c->sp1, N * 4));
+    LEM_TRY(A((void **)&c->pl_front0, N * 4));
+    LEM_TRY(A((void **)&c->pl_front1, N * 4));
+    LEM_TRY(A((void **)&c->pl_U, N * 4));
+    LEM_TRY(A((void **)&c->pl_Uv, N * 4));
+    LEM_TRY(A((void **)&c->pl_stamp, N * 4));
+    LEM_TRY(A(&c->pl_ctl, 256));
+    LEM_CHECK(c, hipMemset(c->pl_stamp, 0, N * 4));
     LEM_TRY(A((void **)&c->cand, N * 4));
     LEM_TRY(A((void **)&c->ws, N * 4));
     LEM_TRY(A((void **)&c->scan_tmp, N * 4));
@@ -190,6 +203,10 @@ extern "C" int lemhip_create(const lem_params *p, int precision, int device_id, 
     LEM_TRY(A((void **)&c->dmask, N));
     long ntiles = ((c->W + 63) / 64) * (long)((c->H + 15) / 16);
     LEM_TRY(A((void **)&c->tile_flags, ntiles * 4));
+    for (int q = 0; q < 3; q++) LEM_TRY(A((void **)&c->dag_list[q], ntiles * 4));
+    LEM_TRY(A((void **)&c->dag_wake, ntiles * 4));
+    LEM_TRY(A(&c->dag_ctl, 256));
+    LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));
     LEM_TRY(A((void **)&c->partials, 4096 * 8));
     LEM_TRY(A((void **)&c->scal, sizeof(LemScalars)));
     LEM_CHECK(c, hipMemset(c->scal, 0, sizeof(LemScalars)));
@@ -452,15 +469,70 @@ extern "C" int lemhip_erosion(lemhip_ctx *c, int iter)
     return LEMHIP_OK;
 }
 
+static int prof_resolve(lemhip_ctx *c)
+{
+    if (!c->prof_on) return LEMHIP_OK;
+    for (int k = 0; k < PROF_COUNT; k++) {
+        ProfSlot &p = c->prof[k];
+        for (int q = 0; q < p.n; q++) {
+            float t = 0.f;
+            LEM_CHECK(c, hipEventElapsedTime(&t, p.a[q], p.b[q]));
+            p.ms += t;
+            p.events++;
+        }
+        p.n = 0;
+    }
+    return LEMHIP_OK;
+}
+
 extern "C" int lemhip_step(lemhip_ctx *c, int iter)
 {
     if (!c) return LEMHIP_E_ARG;
     LEM_CHECK(c, hipSetDevice(c->dev));
+    prof_begin(c, PROF_STEP);
     LEM_TRY(lem_fd_phase(c, iter));
     LEM_TRY(lem_runoff_phase(c));
     LEM_TRY(lem_fa_phase(c, iter));
     LEM_TRY(lem_mod_phase(c, iter));
+    prof_end(c, PROF_STEP);
     LEM_CHECK(c, hipStreamSynchronize(c->s));
+    LEM_TRY(prof_resolve(c));
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_profile_enable(lemhip_ctx *c, int on)
+{
+    if (!c) return LEMHIP_E_ARG;
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    if (on && !c->prof_ready) {
+        for (int k = 0; k < PROF_COUNT; k++)
+            for (int q = 0; q < PROF_MAXP; q++) {
+                LEM_CHECK(c, hipEventCreate(&c->prof[k].a[q]));
+                LEM_CHECK(c, hipEventCreate(&c->prof[k].b[q]));
+            }
+        c->prof_ready = true;
+    }
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    LEM_TRY(prof_resolve(c));
+    for (int k = 0; k < PROF_COUNT; k++) {
+        c->prof[k].ms = 0.0;
+        c->prof[k].launches = 0;
+        c->prof[k].events = 0;
+        c->prof[k].n = 0;
+    }
+    c->prof_on = on != 0 && c->prof_ready;
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_profile_read(lemhip_ctx *c, int n, double *ms, long long *launches)
+{
+    if (!c || !ms || !launches || n < 0) return LEMHIP_E_ARG;
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    LEM_TRY(prof_resolve(c));
+    for (int k = 0; k < n && k < PROF_COUNT; k++) {
+        ms[k] = c->prof[k].ms;
+        launches[k] = c->prof[k].launches;
+    }
     return LEMHIP_OK;
 }
 

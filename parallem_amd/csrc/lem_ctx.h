@@ -6,6 +6,20 @@
 #include "lem_device.h"
 #include "../../include/lemhip.h"
 
+// live timing of selected kernels/phases with HIP events on the context stream
+enum {
+    PROF_FD = 0, PROF_FLOWDIRS, PROF_PLATEAU, PROF_FLOOD, PROF_RUNOFF, PROF_FA, PROF_MOD1, PROF_SED,
+    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_COUNT
+};
+#define PROF_MAXP 64
+struct ProfSlot {
+    hipEvent_t a[PROF_MAXP], b[PROF_MAXP];
+    int n = 0;
+    double ms = 0.0;
+    long launches = 0;
+    long events = 0;
+};
+
 // per-cell status of the dependency engine
 enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2 };
 
@@ -36,6 +50,10 @@ struct lemhip_ctx {
 
     // ---- scratch ----
     float *sp0 = nullptr, *sp1 = nullptr;
+    int *pl_front0 = nullptr, *pl_front1 = nullptr, *pl_U = nullptr, *pl_stamp = nullptr;
+    float *pl_Uv = nullptr;
+    void *pl_ctl = nullptr;
+    int pl_round = 0;
     int *cand = nullptr;       // plateau candidates (also holds pass-2 list)
     int *ws = nullptr;         // watershed labels
     int *scan_tmp = nullptr;   // sink flags / prefix
@@ -55,6 +73,11 @@ struct lemhip_ctx {
     uint8_t *status = nullptr;  // DAG engine status
     uint8_t *dmask = nullptr;   // DAG donor bits (pull order)
     int *tile_flags = nullptr;  // per tile: 1 = finished
+    int *dag_list[3] = {nullptr, nullptr, nullptr};
+    int *dag_wake = nullptr;
+    void *dag_ctl = nullptr;
+    int dag_round = 0;
+    int dag_rounds_last = 0;
     double *partials = nullptr; // deterministic reductions
     LemScalars *scal = nullptr;   // device
     LemScalars *hscal = nullptr;  // pinned host mirror
@@ -65,6 +88,11 @@ struct lemhip_ctx {
     // counters known on the host after each phase
     int h_rounds1 = 0, h_rounds2 = 0, h_nlab = 0, h_nedges = 0, h_sinkcount = 0, h_sinks1 = 0;
     int h_fa_unc = 0, h_sed_unc = 0;
+
+    // profiling
+    bool prof_on = false;
+    bool prof_ready = false;
+    ProfSlot prof[PROF_COUNT];
 
     // tiling of the dependency engine
     int tw = 64, th = 16;
@@ -94,6 +122,23 @@ struct lemhip_ctx {
         int r_ = (x);               \
         if (r_ != LEMHIP_OK) return r_; \
     } while (0)
+
+static inline void prof_begin(lemhip_ctx *c, int id)
+{
+    if (!c->prof_on) return;
+    ProfSlot &p = c->prof[id];
+    if (p.n < PROF_MAXP) (void)hipEventRecord(p.a[p.n], c->s);
+}
+static inline void prof_end(lemhip_ctx *c, int id, long launches = 1)
+{
+    if (!c->prof_on) return;
+    ProfSlot &p = c->prof[id];
+    if (p.n < PROF_MAXP) {
+        (void)hipEventRecord(p.b[p.n], c->s);
+        p.n++;
+        p.launches += launches;
+    }
+}
 
 // phase entry points implemented in the .hip files
 int lem_fd_phase(lemhip_ctx *c, int iter);

@@ -11,6 +11,7 @@
 // in which it completed, so a workgroup trusts another tile's value only when
 // it was written in an earlier launch (kernel boundary = visibility point).
 #include "lem_ctx.h"
+#include <string.h>
 
 #define DAG_PENDING 0x7FFFFFFF
 // status word: DAG_PENDING, r >= 0 done in round r (0 = before the first
@@ -33,59 +34,95 @@ __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
 }
 
 // ------------------------------------------------------------ cell bodies
+//
+// A cell body caches, per tile cell, what its receivers in the same tile will
+// read (LDS), and evaluates one cell once all donors are final.  Donor values
+// come from LDS for donors inside the tile and from HBM for donors in other
+// tiles (completed in an earlier launch).
 
-template <typename T>
+template <typename T, int TW, int TH>
 struct FaCell {
     using FS = typename FaStore<T>::type;
+    static constexpr int NC = TW * TH;
+    static constexpr bool kCanDie = false;
     const T *w;
     FS *fa;
-    // fa = w + max(0, sum of donor fa) (mfd_simple.cu:71-168; prop = 1 for
-    // every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
-    __device__ __forceinline__ void compute(long i, int W, unsigned bits) const;
+    struct Lds {
+        FS val[NC];
+        T w[NC];
+    };
+    __device__ __forceinline__ void preload(Lds &L, int li, long i, bool done) const
+    {
+        L.w[li] = w[i];
+        if (done) L.val[li] = fa[i];
+    }
+    // fa = w + max(0, sum of donor fa in the order E..NE) (mfd_simple.cu:71-168;
+    // prop = 1 for every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
+    __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const;
 };
 
 template <>
-__device__ __forceinline__ void FaCell<double>::compute(long i, int W, unsigned bits) const
+__device__ __forceinline__ void FaCell<double, 64, 16>::compute(Lds &L, int li, long i, int W, unsigned bits, int lr,
+                                                                int lc) const
 {
     double accum = 0.;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         if (bits & (1u << k)) {
             int d = kPULL[k];
-            long n = i + (long)kDY[d] * W + kDX[d];
-            accum += fa[n] * 1.0;
+            int nr = lr + kDY[d], nc = lc + kDX[d];
+            double v = (nr >= 0 && nr < 16 && nc >= 0 && nc < 64) ? L.val[nr * 64 + nc] : fa[i + (long)kDY[d] * W + kDX[d]];
+            accum += v * 1.0;
         }
     }
     if (accum < 0.0) accum = 0.0;
-    double v = w[i];
+    double v = L.w[li];
     v += accum;
     fa[i] = v;
+    L.val[li] = v;
 }
 
 template <>
-__device__ __forceinline__ void FaCell<float>::compute(long i, int W, unsigned bits) const
+__device__ __forceinline__ void FaCell<float, 64, 16>::compute(Lds &L, int li, long i, int W, unsigned bits, int lr,
+                                                               int lc) const
 {
     float2 accum = make_float2(0.f, 0.f);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         if (bits & (1u << k)) {
             int d = kPULL[k];
-            long n = i + (long)kDY[d] * W + kDX[d];
-            accum = ff_add(accum, fa[n]);
+            int nr = lr + kDY[d], nc = lc + kDX[d];
+            float2 v = (nr >= 0 && nr < 16 && nc >= 0 && nc < 64) ? L.val[nr * 64 + nc] : fa[i + (long)kDY[d] * W + kDX[d]];
+            accum = ff_add(accum, v);
         }
     }
     if (accum.x < 0.0f) accum = make_float2(0.f, 0.f);
-    fa[i] = ff_add(make_float2(w[i], 0.f), accum);
+    float2 v = ff_add(make_float2(L.w[li], 0.f), accum);
+    fa[i] = v;
+    L.val[li] = v;
 }
 
-template <typename T>
+template <typename T, int TW, int TH>
 struct SedCell {
+    static constexpr int NC = TW * TH;
+    static constexpr bool kCanDie = true;
     const T *ero, *inci, *geli;
     T *depo, *stone, *fines;
     double *soilT;
     double ddk, dck, dgk;
-    // MFDsedbud body (mfd_accum.cu:63-252)
-    __device__ __forceinline__ void compute(long i, int W, unsigned bits) const
+    struct Lds {
+        T ero[NC], inci[NC], geli[NC], stone[NC];
+    };
+    __device__ __forceinline__ void preload(Lds &L, int li, long i, bool done) const
+    {
+        (void)done;
+        L.ero[li] = ero[i];
+        L.inci[li] = inci[i];
+        L.geli[li] = geli[i];
+        L.stone[li] = stone[i];
+    }
+    // MFDsedbud body (mfd_accum.cu:63-252); donors' stone is their updated value
+    __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const
     {
         T accum_d = 0, accum_c = 0, accum_g = 0;
         T from[8];
@@ -96,14 +133,23 @@ struct SedCell {
             from[k] = 0;
             if (bits & (1u << k)) {
                 int d = kPULL[k];
-                long n = i + (long)kDY[d] * W + kDX[d];
-                accum_d += one * (ero[n] * kd);
-                accum_c += one * (inci[n] * kc);
-                accum_g += one * (geli[n] * kg);
-                from[k] = one * stone[n];
+                int nr = lr + kDY[d], nc = lc + kDX[d];
+                T e, in, g, sn;
+                if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
+                    int nl = nr * TW + nc;
+                    e = L.ero[nl]; in = L.inci[nl]; g = L.geli[nl]; sn = L.stone[nl];
+                } else {
+                    long n = i + (long)kDY[d] * W + kDX[d];
+                    e = ero[n]; in = inci[n]; g = geli[n]; sn = stone[n];
+                }
+                accum_d += one * (e * kd);
+                accum_c += one * (in * kc);
+                accum_g += one * (g * kg);
+                from[k] = one * sn;
             }
         }
-        T dp = ((accum_d + ero[i]) * (T)ddk) + ((accum_c + inci[i]) * (T)dck) + ((accum_g + geli[i]) * (T)dgk);
+        T er = L.ero[li], ic = L.inci[li], ge = L.geli[li];
+        T dp = ((accum_d + er) * (T)ddk) + ((accum_c + ic) * (T)dck) + ((accum_g + ge) * (T)dgk);
         depo[i] = dp;
         double sT = soilT[i];
         double sedthick = sT + (double)dp;
@@ -111,15 +157,18 @@ struct SedCell {
             double old = sT / sedthick;
             double input = 1 - old;
             T propStonefrom = (from[0] + from[1] + from[2] + from[3] + from[4] + from[5] + from[6] + from[7]);
-            T st = (T)((old * (double)stone[i]) + (input * (double)propStonefrom));
-            if (sizeof(T) == 4) st = (T)(((T)old * stone[i]) + ((T)input * propStonefrom));
+            T st;
+            if (sizeof(T) == 8) st = (T)((old * (double)L.stone[li]) + (input * (double)propStonefrom));
+            else st = (T)(((T)old * L.stone[li]) + ((T)input * propStonefrom));
             if (st < (T)0.01) st = (T)0.01;
             else if (st > (T)99.9) st = (T)99.9;
             stone[i] = st;
+            L.stone[li] = st;
             fines[i] = (T)100 - st;
         }
-        double toterosion = (double)ero[i] + (double)inci[i] + (double)geli[i];
-        if (sizeof(T) == 4) toterosion = (double)(ero[i] + inci[i] + geli[i]);
+        double toterosion;
+        if (sizeof(T) == 8) toterosion = (double)er + (double)ic + (double)ge;
+        else toterosion = (double)(er + ic + ge);
         if (toterosion >= sedthick) soilT[i] = 0.0;
         if (sedthick > toterosion) soilT[i] = sedthick - toterosion;
     }
@@ -186,97 +235,203 @@ __global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const
 
 // ------------------------------------------------------------ tiled rounds
 
+// Round bookkeeping: the active tile list of round r is L[r % 3] with count
+// cnt[r % 3]; round r appends the tiles it wakes to L[(r+1) % 3] and clears
+// cnt[(r+2) % 3] (last read by round r-1, next filled by round r+1).
+struct DagCtl {
+    int cnt[3];
+    int flag;
+};
+
+#define DAG_BIG (1 << 24)
+
+// Inside a tile, donors are read from LDS only (values, states, counters);
+// HBM copies are read by other tiles in later launches.  So ordering a
+// resolved cell's LDS writes before the counter update that can hand it to
+// another thread needs only the LDS queue drained, not the global stores.
+__device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// One round over the active tiles.  Inside a tile every cell gets a counter of
+// donors still unresolved in the tile; cells whose donors in other tiles are
+// not all final from earlier launches are parked for a later round.  A thread
+// resolving a cell decrements its receivers in the tile, and the thread that
+// brings a receiver's counter to zero continues along it, so a dependency
+// chain runs inside the tile at LDS speed with no barrier per link.  Extra
+// ready receivers (MFD) go to an LDS spill list processed after a barrier.
+// A cell with a dead donor dies (sediment blocking, D9).
 template <int TW, int TH, class Cell>
 __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
-                                                   int *__restrict__ st, int *__restrict__ tile_done, int *progress)
+                                                   const uint8_t *__restrict__ fdv, int *__restrict__ st,
+                                                   int *__restrict__ tile_done, const int *__restrict__ act,
+                                                   int *__restrict__ next, int *__restrict__ wake, DagCtl *ctl)
 {
     constexpr int NC = TW * TH;
     constexpr int PER = NC / 256;
+    __shared__ int lcnt[NC];
     __shared__ uint8_t lst[NC];
+    __shared__ uint8_t ldm[NC];
+    __shared__ uint8_t lfd[NC];
+    __shared__ typename Cell::Lds lds;
+    __shared__ int spill[2][NC];
+    __shared__ int nspill[2];
     __shared__ int any_pending;
-    int tile = blockIdx.x;
-    if (tile_done[tile]) return;
-    int ntx = (W + TW - 1) / TW;
-    int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
-    if (threadIdx.x == 0) any_pending = 0;
-    // local state: 0 pending, 1 done, 2 dead (off-grid slots count as done)
-    uint8_t mybits[PER];
-    long myidx[PER];
-    bool myact[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int li = threadIdx.x + q * 256;
-        int r = ty0 + li / TW, c = tx0 + li % TW;
-        bool in = r < H && c < W;
-        long i = (long)r * W + c;
-        myidx[q] = i;
-        uint8_t s = 1;
-        uint8_t b = 0;
-        if (in) {
-            int v = st[i];
-            s = (v == DAG_PENDING) ? 0 : (v >= 0 ? 1 : 2);
-            b = dm[i];
-        }
-        lst[li] = s;
-        mybits[q] = b;
-        myact[q] = in && s == 0;
-    }
-    __syncthreads();
-    bool progressed = false;
-    for (;;) {
-        uint8_t nxt[PER];
-        bool any = false;
+    const int n = ctl->cnt[round % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(round + 2) % 3] = 0;
+    const int ntx = (W + TW - 1) / TW;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const int tile = act[t];
+        if (tile_done[tile]) continue;
+        const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
+        __syncthreads();
+        if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; }
+        // 1. load states and the tile's cached values
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            nxt[q] = 0;
-            if (!myact[q]) continue;
             int li = threadIdx.x + q * 256;
+            int r = ty0 + li / TW, c = tx0 + li % TW;
+            uint8_t sv = ST_DONE;
+            uint8_t b = 0, f = 0;
+            if (r < H && c < W) {
+                long i = (long)r * W + c;
+                int v = st[i];
+                sv = (v == DAG_PENDING) ? ST_PENDING : (v >= 0 ? ST_DONE : ST_DEAD);
+                cell.preload(lds, li, i, sv == ST_DONE);
+                b = dm[i];
+                f = fdv[i];
+            }
+            lst[li] = sv;
+            ldm[li] = b;
+            lfd[li] = f;
+            lcnt[li] = DAG_BIG;
+        }
+        __syncthreads();
+        // 2. counters: donors unresolved in the tile; park cells waiting on other tiles
+        int seeds = 0;     // bit q: cell q of this thread starts a walk
+        int deadseeds = 0; // bit q: ... and it dies
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int li = threadIdx.x + q * 256;
+            if (lst[li] != ST_PENDING) continue;
             int lr = li / TW, lc = li % TW;
-            unsigned bits = mybits[q];
-            bool ready = true, dead = false;
+            long i = (long)(ty0 + lr) * W + (tx0 + lc);
+            unsigned bits = ldm[li];
+            int nin = 0;
+            bool dead = false, parked = false;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 if (!(bits & (1u << k))) continue;
                 int d = kPULL[k];
                 int nr = lr + kDY[d], nc = lc + kDX[d];
-                uint8_t s;
                 if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
-                    s = lst[nr * TW + nc];
+                    uint8_t s2 = lst[nr * TW + nc];
+                    if (s2 == ST_PENDING) nin++;
+                    else if (s2 == ST_DEAD) dead = true;
                 } else {
-                    int v = st[myidx[q] + (long)kDY[d] * W + kDX[d]];
-                    if (v == DAG_PENDING) s = 0;
-                    else if (v >= 0) s = (v < round) ? 1 : 0;
-                    else s = ((-v - 1) < round) ? 2 : 0;
+                    int v = st[i + (long)kDY[d] * W + kDX[d]];
+                    if (v == DAG_PENDING) parked = true;
+                    else if (v >= 0) { if (v >= round) parked = true; }
+                    else { if ((-v - 1) < round) dead = true; else parked = true; }
                 }
-                if (s == 2) dead = true;
-                if (s != 1) ready = false;
             }
-            if (dead) nxt[q] = 2;
-            else if (ready) {
-                cell.compute(myidx[q], W, bits);
-                nxt[q] = 1;
-            }
-            any |= (nxt[q] != 0);
+            if (dead) { lcnt[li] = DAG_BIG; seeds |= 1 << q; deadseeds |= 1 << q; }
+            else if (!parked) { lcnt[li] = nin; if (nin == 0) seeds |= 1 << q; }
         }
         __syncthreads();
+        // 3. walks
+        bool wake_any = false;
+        int pass = 0;
+        for (;;) {
+            int nwork = (pass == 0) ? 0 : nspill[(pass - 1) & 1];
+            if (pass > 0 && nwork == 0) break;
+            for (int w0 = 0;; w0++) {
+                int cur = -1;
+                bool seed_dead = false;
+                if (pass == 0) {
+                    if (w0 >= PER) break;
+                    if (!(seeds & (1 << w0))) continue;
+                    cur = threadIdx.x + w0 * 256;
+                    seed_dead = (deadseeds >> w0) & 1;
+                } else {
+                    int idx = threadIdx.x + w0 * 256;
+                    if (idx >= nwork) break;
+                    cur = spill[(pass - 1) & 1][idx];
+                }
+                while (cur >= 0) {
+                    int lr = cur / TW, lc = cur % TW;
+                    long i = (long)(ty0 + lr) * W + (tx0 + lc);
+                    unsigned bits = ldm[cur];
+                    lds_order();
+                    // a cell reached through its counter has every donor in other
+                    // tiles final (else it was parked); only donors in this tile can
+                    // have died since the counters were set
+                    bool dead = seed_dead;
+                    seed_dead = false;
+                    if (Cell::kCanDie && !dead) {
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            if (nxt[q]) {
-                int li = threadIdx.x + q * 256;
-                lst[li] = nxt[q];
-                st[myidx[q]] = (nxt[q] == 1) ? round : -(round + 1);
-                myact[q] = false;
+                        for (int k = 0; k < 8; k++) {
+                            if (!(bits & (1u << k))) continue;
+                            int d = kPULL[k];
+                            int nr = lr + kDY[d], nc = lc + kDX[d];
+                            if (nr >= 0 && nr < TH && nc >= 0 && nc < TW && lst[nr * TW + nc] == ST_DEAD) dead = true;
+                        }
+                    }
+                    if (dead) {
+                        lst[cur] = ST_DEAD;
+                        st[i] = -(round + 1);
+                    } else {
+                        cell.compute(lds, cur, i, W, bits, lr, lc);
+                        lst[cur] = ST_DONE;
+                        st[i] = round;
+                    }
+                    lds_order();
+                    int nxt = -1;
+                    unsigned f = lfd[cur];
+#pragma unroll
+                    for (int d = 0; d < 8; d++) {
+                        if (!(f & kCODE[d])) continue;
+                        int nr = lr + kDY[d], nc = lc + kDX[d];
+                        if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
+                            int rl = nr * TW + nc;
+                            if (atomicSub(&lcnt[rl], 1) == 1) {
+                                if (nxt < 0) nxt = rl;
+                                else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = rl;
+                            }
+                        } else {
+                            int gy = ty0 + nr, gx = tx0 + nc;
+                            if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+                                int nt = (gy / TH) * ntx + (gx / TW);
+                                if (!tile_done[nt] && atomicExch(&wake[nt], round + 1) != round + 1)
+                                    next[atomicAdd(&ctl->cnt[(round + 1) % 3], 1)] = nt;
+                                wake_any = true;
+                            }
+                        }
+                    }
+                    cur = nxt;
+                }
             }
+            __syncthreads();
+            if (threadIdx.x == 0) nspill[(pass + 1) & 1] = 0;
+            __syncthreads();
+            pass++;
         }
-        progressed |= any;
-        if (!__syncthreads_or(any)) break;
-    }
-    bool pend = false;
+        // 4. bookkeeping
+        bool pend = false;
 #pragma unroll
-    for (int q = 0; q < PER; q++) pend |= myact[q];
-    if (pend) any_pending = 1;
-    if (__syncthreads_or(progressed) && threadIdx.x == 0) *progress = 1;
-    if (threadIdx.x == 0 && !any_pending) tile_done[tile] = 1;
+        for (int q = 0; q < PER; q++) pend |= (lst[threadIdx.x + q * 256] == ST_PENDING);
+        if (pend) any_pending = 1;
+        int prog = __syncthreads_or(seeds != 0);
+        (void)wake_any;
+        if (threadIdx.x == 0) {
+            if (prog) ctl->flag = 1;
+            if (!any_pending) tile_done[tile] = 1;
+        }
+    }
+}
+
+__global__ void k_iota(int n, int *a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
 }
 
 __global__ void k_count_pending(long N, const uint8_t *mask, const int *st, int *out)
@@ -308,16 +463,38 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed)
     int ntiles = ((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
     LEM_CHECK(c, hipMemsetAsync(c->tile_flags, 0, sizeof(int) * ntiles, c->s));
     int *st = reinterpret_cast<int *>(c->scan_tmp);
-    for (int round = 1;; ++round) {
-        LEM_CHECK(c, hipMemsetAsync(&c->scal->flag, 0, sizeof(int), c->s));
-        k_dag_round<TW, TH, Cell><<<ntiles, 256, 0, c->s>>>(c->W, c->H, round, cell, c->dmask, st, c->tile_flags,
-                                                            &c->scal->flag);
-        LEM_LAUNCHED(c);
-        LEM_CHECK(c, hipMemcpyAsync(&c->hint[2], &c->scal->flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
-        LEM_CHECK(c, hipStreamSynchronize(c->s));
-        if (!c->hint[2]) break;
-        if (round > c->N + 2) { c->err = "dependency rounds did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    int *L[3] = {c->dag_list[0], c->dag_list[1], c->dag_list[2]};
+    DagCtl *ctl = reinterpret_cast<DagCtl *>(c->dag_ctl);
+    int r = c->dag_round;
+    if (r > (1 << 29)) {  // wake stamps and status words hold round numbers
+        LEM_CHECK(c, hipMemsetAsync(c->dag_wake, 0, sizeof(int) * ntiles, c->s));
+        r = 0;
     }
+    k_iota<<<(ntiles + 255) / 256, 256, 0, c->s>>>(ntiles, L[(r + 1) % 3]);
+    LEM_LAUNCHED(c);
+    int h[4] = {0, 0, 0, 0};
+    h[(r + 1) % 3] = ntiles;
+    memcpy(&c->hint[16], h, sizeof(h));
+    LEM_CHECK(c, hipMemcpyAsync(ctl, &c->hint[16], sizeof(DagCtl), hipMemcpyHostToDevice, c->s));
+    const int BATCH = 16;
+    int G = ntiles < 512 ? ntiles : 512;
+    long guard = 0;
+    for (;;) {
+        LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));
+        for (int q = 0; q < BATCH; q++) {
+            ++r;
+            k_dag_round<TW, TH, Cell><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, c->tile_flags,
+                                                           L[r % 3], L[(r + 1) % 3], c->dag_wake, ctl);
+        }
+        LEM_LAUNCHED(c);
+        LEM_CHECK(c, hipMemcpyAsync(&c->hint[20], &ctl->flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        LEM_CHECK(c, hipStreamSynchronize(c->s));
+        if (!c->hint[20]) break;
+        guard += 1;  // batches that made progress; each completes at least one cell
+        if (guard > c->N + 2) { c->err = "dependency rounds did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    c->dag_rounds_last = r - c->dag_round;
+    c->dag_round = r;
     LEM_CHECK(c, hipMemsetAsync(&c->scal->flag, 0, sizeof(int), c->s));
     k_count_pending<<<grid1d_dag(c->N), 256, 0, c->s>>>(c->N, c->mask, st, &c->scal->flag);
     LEM_LAUNCHED(c);
@@ -334,7 +511,7 @@ int lem_fa_dag(lemhip_ctx *c)
     int *st = reinterpret_cast<int *>(c->scan_tmp);
     k_fa_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->dmask, st);
     LEM_LAUNCHED(c);
-    FaCell<T> cell{reinterpret_cast<const T *>(c->w), reinterpret_cast<typename FaStore<T>::type *>(c->fa)};
+    FaCell<T, 64, 16> cell{reinterpret_cast<const T *>(c->w), reinterpret_cast<typename FaStore<T>::type *>(c->fa)};
     return run_rounds<64, 16>(c, cell, &c->h_fa_unc);
 }
 
@@ -348,7 +525,7 @@ int lem_sed_dag(lemhip_ctx *c)
     double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
     if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407
     if (dck > 0.9) dck = 0.9;
-    SedCell<T> cell{reinterpret_cast<const T *>(c->ero), reinterpret_cast<const T *>(c->inci),
+    SedCell<T, 64, 16> cell{reinterpret_cast<const T *>(c->ero), reinterpret_cast<const T *>(c->inci),
                     reinterpret_cast<const T *>(c->geli), reinterpret_cast<T *>(c->depo),
                     reinterpret_cast<T *>(c->stone), reinterpret_cast<T *>(c->fines), c->soilT, ddk, dck, dgk};
     return run_rounds<64, 16>(c, cell, &c->h_sed_unc);

@@ -20,6 +20,32 @@ __device__ __forceinline__ int wave_append(int *counter, bool pred)
     return pred ? base + __popcll(b & ((1ULL << lane) - 1ULL)) : -1;
 }
 
+// one global atomic per workgroup (block size a multiple of 64, at most 1024);
+// every thread of the block must call it
+__device__ __forceinline__ int block_append(int *counter, bool pred, int *s_wave, int *s_base)
+{
+    unsigned long long b = __ballot(pred);
+    int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    int lane = tid & 63;
+    int wave = tid >> 6;
+    int nwaves = (blockDim.x * blockDim.y) >> 6;
+    if (lane == 0) s_wave[wave] = __popcll(b);
+    __syncthreads();
+    if (tid == 0) {
+        int tot = 0;
+        for (int w = 0; w < nwaves; w++) {
+            int cnt = s_wave[w];
+            s_wave[w] = tot;
+            tot += cnt;
+        }
+        *s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    int pos = *s_base + s_wave[wave] + __popcll(b & ((1ULL << lane) - 1ULL));
+    __syncthreads();
+    return pred ? pos : -1;
+}
+
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v, T *sh)
 {
@@ -82,97 +108,154 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
                                                   long plane, uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
                                                   int *__restrict__ cand, int *__restrict__ ncand)
 {
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
     int icol = blockIdx.x * blockDim.x + threadIdx.x;
-    int irow = blockIdx.y * blockDim.y + threadIdx.y;
-    bool inside = icol < W && irow < H;
-    long self = (long)irow * W + icol;
-    bool active = inside && mask[self];
-    int aspect = 0, aspectsfd = 0;
-    if (active) {
-        double zc = z[self];
-        double smax = 0.0;
-        int up = 0, flat = 0;
+#pragma unroll 1
+    for (int rg = 0; rg < 4; rg++) {
+        int irow = blockIdx.y * 16 + rg * 4 + threadIdx.y;
+        bool inside = icol < W && irow < H;
+        long self = (long)irow * W + icol;
+        bool active = inside && mask[self];
+        int aspect = 0, aspectsfd = 0;
+        if (active) {
+            double zc = z[self];
+            double smax = 0.0;
+            int up = 0, flat = 0;
 #pragma unroll
-        for (int d = 0; d < 8; d++) {
-            int cx = icol + kDX[d], cy = irow + kDY[d];
-            if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
-                float dc = (d & 1) ? 1.41f : 1.0f;
-                float den = (float)cell_size * dc;
-                double zn = z[(long)cy * W + cx];
-                double stemp = (zc - zn) / (double)den;
-                if (zn == -9999) stemp = -0.0000001;
-                if ((zc > zn) && (zn != -9999)) {
-                    if (slopes) slopes[d * plane + self] = (TS)stemp;
-                    if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
-                    aspect += kCODE[d];
+            for (int d = 0; d < 8; d++) {
+                int cx = icol + kDX[d], cy = irow + kDY[d];
+                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+                    float dc = (d & 1) ? 1.41f : 1.0f;
+                    float den = (float)cell_size * dc;
+                    double zn = z[(long)cy * W + cx];
+                    double stemp = (zc - zn) / (double)den;
+                    if (zn == -9999) stemp = -0.0000001;
+                    if ((zc > zn) && (zn != -9999)) {
+                        if (slopes) slopes[d * plane + self] = (TS)stemp;
+                        if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
+                        aspect += kCODE[d];
+                    }
+                    if (zc < zn) up++;
+                    if (zc == zn) flat++;
                 }
-                if (zc < zn) up++;
-                if (zc == zn) flat++;
             }
+            if (up > 7) { aspect = 0; aspectsfd = 0; }
+            if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
+            if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
         }
-        if (up > 7) { aspect = 0; aspectsfd = 0; }
-        if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
-        if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
+        if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; }
+        int slot = block_append(ncand, active && aspect == 0 && aspectsfd == 0, s_wave, &s_base);
+        if (slot >= 0) cand[slot] = (int)self;
     }
-    if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; }
-    int slot = wave_append(ncand, active && aspect == 0 && aspectsfd == 0);
-    if (slot >= 0) cand[slot] = (int)self;
 }
 
 // ------------------------------------------------------ A3/A4: plateaus
 
-__global__ void k_sp_init(long N, float *sp0, float *sp1)
+__global__ void k_sp_init(long N, float *sp)
 {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        sp0[i] = 0.0f;
-        sp1[i] = 0.0f;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) sp[i] = 0.0f;
+}
+
+// candidates start at 2e7 (newflow.cu:252); the first round examines all of them
+__global__ void k_sp_seed(const int *cand, int n, float *sp, int *front)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) { sp[cand[k]] = 20000000.0f; front[k] = cand[k]; }
+}
+
+// Counters of the frontier machinery, indexed by round: frontier F[r % 2]
+// (count cnt[r % 2]), update list U (count cnt[2 + r % 2]).
+struct PlatCtl {
+    int cnt[4];
+    int changed_rounds;
+    int flag;
+};
+
+// route_plateausMFD (newflow.cu:264-350), one synchronous round (D1), part 1:
+// cells of the frontier read only round r-1 distances (nothing writes sp in
+// this kernel); each strict improvement in the scan marks that slope and
+// takes its direction; improved cells are queued with their new distance.
+// Only a cell with an equal-height neighbour that changed in round r-1 can
+// improve in round r, so the frontier holds exactly those cells.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_plat_scan(int W, int H, int r, const int *__restrict__ front,
+                                                   const double *__restrict__ z, const float *__restrict__ sp,
+                                                   uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
+                                                   TS *__restrict__ slopes, long plane, int *__restrict__ U,
+                                                   float *__restrict__ Uv, PlatCtl *ctl)
+{
+    int n = ctl->cnt[r & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(r + 1) & 1] = 0;  // next frontier, filled by part 2
+    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {
+        int k = k0 + threadIdx.x;
+        bool changed = false;
+        float min_distance = 0.f;
+        long self = 0;
+        if (k < n) {
+            self = front[k];
+            int irow = (int)(self / W), icol = (int)(self % W);
+            min_distance = sp[self];
+            double zc = z[self];
+            int code = -1;
+            for (int d = 0; d < 8; d++) {
+                int cx = icol + kDX[d], cy = irow + kDY[d];
+                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+                    float dc = (d & 1) ? 1.41f : 1.0f;
+                    long nb = (long)cy * W + cx;
+                    float dis = sp[nb] + dc;
+                    if (zc == z[nb] && min_distance > dis) {
+                        min_distance = dis;
+                        changed = true;
+                        code = kCODE[d];
+                        if (slopes) slopes[d * plane + self] = (TS)0.0001;
+                    }
+                }
+            }
+            if (changed) { fd[self] = (uint8_t)code; SFD[self] = (uint8_t)code; }
+        }
+        int slot = wave_append(&ctl->cnt[2 + (r & 1)], changed);
+        if (slot >= 0) { U[slot] = (int)self; Uv[slot] = min_distance; }
     }
 }
 
-__global__ void k_sp_seed(const int *cand, int n, float *sp0, float *sp1)
+// part 2: publish the round's new distances and queue, once, every
+// equal-height candidate neighbour of a changed cell for round r+1.  A cell is
+// a candidate exactly when its distance is nonzero, which no update changes,
+// so the test does not depend on the order of the writes in this kernel.
+__global__ void __launch_bounds__(256) k_plat_next(int W, int H, int r, const int *__restrict__ U,
+                                                   const float *__restrict__ Uv, const double *__restrict__ z,
+                                                   float *__restrict__ sp, int *__restrict__ stamp,
+                                                   int *__restrict__ front_next, PlatCtl *ctl)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) { sp0[cand[k]] = 20000000.0f; sp1[cand[k]] = 20000000.0f; }
-}
-
-// route_plateausMFD (newflow.cu:264-350) as one synchronous round (D1): every
-// read sees the previous round's distances; each strict improvement in the
-// scan marks that slope and takes its direction.
-template <typename TS>
-__global__ void __launch_bounds__(256) k_plateau_round(int W, int H, const int *__restrict__ cand, int n,
-                                                       const double *__restrict__ z, const float *__restrict__ sp_prev,
-                                                       float *__restrict__ sp_next, uint8_t *__restrict__ fd,
-                                                       uint8_t *__restrict__ SFD, TS *__restrict__ slopes, long plane,
-                                                       int *flag)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    long self = cand[k];
-    int irow = (int)(self / W), icol = (int)(self % W);
-    float min_distance = sp_prev[self];
-    if (min_distance == 0) { sp_next[self] = min_distance; return; }
-    double zc = z[self];
-    bool changed = false;
-    int code = -1;
-    for (int d = 0; d < 8; d++) {
-        int cx = icol + kDX[d], cy = irow + kDY[d];
-        if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
-            float dc = (d & 1) ? 1.41f : 1.0f;
-            long nb = (long)cy * W + cx;
-            float dis = sp_prev[nb] + dc;
-            if (zc == z[nb] && min_distance > dis) {
-                min_distance = dis;
-                changed = true;
-                code = kCODE[d];
-                if (slopes) slopes[d * plane + self] = (TS)0.0001;
+    int m = ctl->cnt[2 + (r & 1)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->cnt[2 + ((r + 1) & 1)] = 0;
+        if (m > 0) { ctl->changed_rounds++; ctl->flag = 1; }
+    }
+    for (int k0 = blockIdx.x * blockDim.x; k0 < m; k0 += gridDim.x * blockDim.x) {
+        int k = k0 + threadIdx.x;
+        int picks[8];
+        int np = 0;
+        if (k < m) {
+            long self = U[k];
+            sp[self] = Uv[k];
+            int irow = (int)(self / W), icol = (int)(self % W);
+            double zc = z[self];
+            for (int d = 0; d < 8; d++) {
+                int cx = icol + kDX[d], cy = irow + kDY[d];
+                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+                    long nb = (long)cy * W + cx;
+                    if (z[nb] == zc && sp[nb] != 0.0f && atomicExch(&stamp[nb], r + 1) != r + 1)
+                        picks[np++] = (int)nb;
+                }
             }
         }
-    }
-    sp_next[self] = min_distance;
-    if (changed) {
-        fd[self] = (uint8_t)code;
-        SFD[self] = (uint8_t)code;
-        *flag = 1;
+        for (int q = 0; q < 8; q++) {
+            bool have = q < np;
+            int slot = wave_append(&ctl->cnt[(r + 1) & 1], have);
+            if (have) front_next[slot] = picks[q];
+        }
     }
 }
 
@@ -318,13 +401,16 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
     }
 }
 
-__global__ void k_edges_compact(long cap, const unsigned long long *keys, const unsigned long long *vals,
-                                uint32_t *elo, uint32_t *ehi, unsigned long long *ew, int *count, uint8_t *seen)
+__global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned long long *keys,
+                                                       const unsigned long long *vals, uint32_t *elo, uint32_t *ehi,
+                                                       unsigned long long *ew, int *count, uint8_t *seen)
 {
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < cap; i += (long)gridDim.x * blockDim.x) {
         unsigned long long k = keys[i];
         bool have = k != HASH_EMPTY;
-        int slot = wave_append(count, have);
+        int slot = block_append(count, have, s_wave, &s_base);
         if (have) {
             uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)(k & 0xffffffffu);
             elo[slot] = lo;
@@ -505,9 +591,11 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     int W = c->W, H = c->H;
     LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
     dim3 blk(64, 4);
-    dim3 grd((W + 63) / 64, (H + 3) / 4);
+    dim3 grd((W + 63) / 64, (H + 15) / 16);
+    prof_begin(c, PROF_FLOWDIRS);
     k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, N, c->fd, c->SFD,
                                           c->cand, &c->scal->ncand);
+    prof_end(c, PROF_FLOWDIRS);
     LEM_LAUNCHED(c);
     LEM_CHECK(c, hipMemcpyAsync(&c->hscal->ncand, &c->scal->ncand, sizeof(int), hipMemcpyDeviceToHost, c->s));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
@@ -515,22 +603,46 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     *ncand_out = n;
     *rounds = 0;
     if (n == 0) return LEMHIP_OK;
-    k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, c->sp0, c->sp1);
-    k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(c->cand, n, c->sp0, c->sp1);
-    LEM_LAUNCHED(c);
-    float *a = c->sp0, *b = c->sp1;
-    for (;;) {
-        LEM_TRY(clear_flag(c));
-        k_plateau_round<TS><<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, n, zs, a, b, c->fd, c->SFD,
-                                                                      slopes, N, &c->scal->flag);
-        LEM_LAUNCHED(c);
-        int f = 0;
-        LEM_TRY(read_flag(c, &f));
-        float *t = a; a = b; b = t;
-        if (!f) break;
-        ++*rounds;
-        if (*rounds > n + 2) { c->err = "plateau routing did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    // frontier machinery: sp0 = distances, cand = round-1 frontier copy source,
+    // ws/scan_tmp hold the two frontiers, z2-sized scratch is not touched here
+    float *sp = c->sp0;
+    int *F[2] = {c->pl_front0, c->pl_front1};
+    PlatCtl *ctl = reinterpret_cast<PlatCtl *>(c->pl_ctl);
+    int r0 = c->pl_round;
+    if (r0 > (1 << 29)) {  // stamps hold round numbers: restart them well before they wrap
+        LEM_CHECK(c, hipMemsetAsync(c->pl_stamp, 0, N * sizeof(int), c->s));
+        r0 = 0;
     }
+    k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, sp);
+    k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(c->cand, n, sp, F[(r0 + 1) & 1]);
+    LEM_LAUNCHED(c);
+    c->hint[8] = 0; c->hint[9] = 0; c->hint[10] = 0; c->hint[11] = 0;
+    c->hint[8 + ((r0 + 1) & 1)] = n;
+    c->hint[12] = 0; c->hint[13] = 0;
+    LEM_CHECK(c, hipMemcpyAsync(ctl, &c->hint[8], sizeof(PlatCtl), hipMemcpyHostToDevice, c->s));
+    prof_begin(c, PROF_PLATEAU);
+    int r = r0;
+    const int BATCH = 32;
+    const int G = 128;
+    for (int batch = 0;; ++batch) {
+        LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));
+        for (int q = 0; q < BATCH; q++) {
+            ++r;
+            // round 1 scans every candidate; later frontiers are thin wavefronts
+            int g = (r == r0 + 1) ? grid1d(n, 256, 4096) : G;
+            k_plat_scan<TS><<<g, 256, 0, c->s>>>(W, H, r, F[r & 1], zs, sp, c->fd, c->SFD, slopes, N, c->pl_U,
+                                                 c->pl_Uv, ctl);
+            k_plat_next<<<g, 256, 0, c->s>>>(W, H, r, c->pl_U, c->pl_Uv, zs, sp, c->pl_stamp, F[(r + 1) & 1], ctl);
+        }
+        LEM_LAUNCHED(c);
+        LEM_CHECK(c, hipMemcpyAsync(&c->hint[8], ctl, sizeof(PlatCtl), hipMemcpyDeviceToHost, c->s));
+        LEM_CHECK(c, hipStreamSynchronize(c->s));
+        if (!c->hint[13]) break;
+        if ((long)batch > (long)n + 2) { c->err = "plateau routing did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    c->pl_round = r;
+    *rounds = c->hint[12];
+    prof_end(c, PROF_PLATEAU, *rounds + 1);
     return LEMHIP_OK;
 }
 
@@ -629,24 +741,25 @@ static int flooding(lemhip_ctx *c, int ncand1)
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     int nlab = c->hint[0] + c->hint[1];
     c->h_nlab = nlab;
-    for (int it = 0;; ++it) {
+    for (int it = 0;; it += 4) {
         LEM_TRY(clear_flag(c));
-        k_sink_merge<<<grid1d(ncand1, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, ncand1, c->SFD, c->dem, c->ws,
-                                                                     &c->scal->flag);
+        for (int q = 0; q < 4; q++)
+            k_sink_merge<<<grid1d(ncand1, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, ncand1, c->SFD, c->dem,
+                                                                         c->ws, &c->scal->flag);
         LEM_LAUNCHED(c);
         int f = 0;
         LEM_TRY(read_flag(c, &f));
         if (!f) break;
-        if (it > ncand1 + 2) { c->err = "sink merge did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        if (it / 4 > ncand1 + 2) { c->err = "sink merge did not converge"; return LEMHIP_E_NOT_CONVERGED; }
     }
-    for (int it = 0;; ++it) {
+    for (int it = 0;; it += 4) {
         LEM_TRY(clear_flag(c));
-        k_jump<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, &c->scal->flag);
+        for (int q = 0; q < 4; q++) k_jump<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, &c->scal->flag);
         LEM_LAUNCHED(c);
         int f = 0;
         LEM_TRY(read_flag(c, &f));
         if (!f) break;
-        if (it > 64) { c->err = "watershed pointer jumping did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        if (it > 128) { c->err = "watershed pointer jumping did not converge"; return LEMHIP_E_NOT_CONVERGED; }
     }
     // edges
     LEM_TRY(ensure_labels(c, (long)nlab + 1));
@@ -679,16 +792,17 @@ static int flooding(lemhip_ctx *c, int ncand1)
     int ne = c->hscal->nedges;
     c->h_nedges = ne;
     c->h_sinkcount = c->hscal->sinkcount;
-    for (int it = 0;; ++it) {
+    for (int it = 0;; it += 16) {
         LEM_TRY(clear_flag(c));
         if (ne > 0)
-            k_minimax_relax<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->ew, c->rmm,
-                                                                        &c->scal->flag);
+            for (int q = 0; q < 16; q++)
+                k_minimax_relax<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->ew, c->rmm,
+                                                                            &c->scal->flag);
         LEM_LAUNCHED(c);
         int f = 0;
         LEM_TRY(read_flag(c, &f));
         if (!f) break;
-        if (it > nlab + 2) { c->err = "spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        if (it / 16 > nlab + 2) { c->err = "spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
     }
     LEM_TRY(clear_flag(c));
     if (ne > 0) k_fb_unreached<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->rmm, &c->scal->flag);
@@ -707,6 +821,7 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
     (void)iter;
     long N = c->N;
     // lowest cell for the outlet check, taken on the unadjusted surface
+    prof_begin(c, PROF_FD);
     unsigned long long init = ord_enc(1000.0);
     LEM_CHECK(c, hipMemcpyAsync(&c->scal->lowest_enc, &init, 8, hipMemcpyHostToDevice, c->s));
     k_lowest<<<grid1d(N, 256, 2048), 256, 0, c->s>>>(N, c->dem, c->mask, &c->scal->lowest_enc);
@@ -729,7 +844,9 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
     c->h_nedges = 0;
     c->h_sinkcount = sinks;
     if (sinks > 0) {
+        prof_begin(c, PROF_FLOOD);
         LEM_TRY(flooding(c, n1));
+        prof_end(c, PROF_FLOOD);
         int n2 = 0, r2 = 0;
         LEM_TRY(directions_and_plateaus<TS>(c, c->z2, (TS *)nullptr, &n2, &r2));
         c->h_rounds2 = r2;
@@ -737,6 +854,7 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
     k_merge<<<grid1d(N), 256, 0, c->s>>>(N, c->p.flowdirectiontype, c->mask, c->fd, c->SFD, c->scal);
     k_outlet_adjust<<<1, 1, 0, c->s>>>(c->dem, c->scal);
     LEM_LAUNCHED(c);
+    prof_end(c, PROF_FD);
     return LEMHIP_OK;
 }
 

@@ -365,8 +365,10 @@ template <typename T>
 static int runoff_t(lemhip_ctx *c)
 {
     using FS = typename FaS<T>::type;
+    prof_begin(c, PROF_RUNOFF);
     k_runoff<T><<<grid1d(c->N), 256, 0, c->s>>>(c->N, c->mask, (const T *)c->rain, (const T *)c->stone,
                                                  (const T *)c->TotB, (T *)c->soilM, (T *)c->w, (FS *)c->fa);
+    prof_end(c, PROF_RUNOFF);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
 }
@@ -380,7 +382,9 @@ template <typename T>
 static int fa_t(lemhip_ctx *c, int iter)
 {
     using FS = typename FaS<T>::type;
+    prof_begin(c, PROF_FA);
     LEM_TRY(lem_fa_dag<T>(c));
+    prof_end(c, PROF_FA, c->dag_rounds_last);
     // FA_max and the first-iteration outlet rule (mfd_simple.cu:332-357)
     unsigned long long *mx = reinterpret_cast<unsigned long long *>(c->partials + RED_BLOCKS);
     int *idx = reinterpret_cast<int *>(c->partials + RED_BLOCKS + 1);
@@ -409,22 +413,30 @@ static int mod_t(lemhip_ctx *c, int iter)
     using FS = typename FaS<T>::type;
     long N = c->N;
     int g = grid1d(N, 256, 16384);
+    prof_begin(c, PROF_MOD1);
     k_mod1<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa, (const T *)c->fines,
                                    (const T *)c->nut, (const T *)c->stone, (const T *)c->TotB, c->soilT,
                                    (const T *)c->temp, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci, (T *)c->geli,
                                    (T *)c->depo);
+    prof_end(c, PROF_MOD1);
     LEM_LAUNCHED(c);
+    prof_begin(c, PROF_REDUCE);
     LEM_TRY(dsum<T>(c, c->ero, &c->scal->totE));
     LEM_TRY(dsum<T>(c, c->inci, &c->scal->totI));
     LEM_TRY(dsum<T>(c, c->geli, &c->scal->totG));
+    prof_end(c, PROF_REDUCE, 3);
+    prof_begin(c, PROF_SED);
     LEM_TRY(lem_sed_dag<T>(c));
+    prof_end(c, PROF_SED, c->dag_rounds_last);
     LEM_TRY(dsum<T>(c, c->depo, &c->scal->totD));
     k_outlet_pre<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    prof_begin(c, PROF_MOD2);
     k_mod2<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const FS *)c->fa, (const T *)c->ero, (const T *)c->inci,
                                    (const T *)c->geli, (const T *)c->depo, (const T *)c->temp, (const T *)c->rain,
                                    (const T *)c->last_temp, c->soilT, (T *)c->stone, (T *)c->fines, (T *)c->soilM,
                                    (T *)c->soilB, (T *)c->nut, (T *)c->TotB, c->dem, (T *)c->dz, (T *)c->wC,
                                    (T *)c->wP, c->scal);
+    prof_end(c, PROF_MOD2);
     k_outlet_post<<<1, 1, 0, c->s>>>(c->dem, c->scal);
     LEM_LAUNCHED(c);
     LEM_TRY(dsum<T>(c, c->wC, &c->scal->totwC));

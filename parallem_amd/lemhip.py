@@ -38,7 +38,11 @@ EXPORTS = [
     "lemhip_download", "lemhip_get_outlet", "lemhip_set_outlet", "lemhip_flow_direction", "lemhip_runoff",
     "lemhip_flow_accum", "lemhip_erosion", "lemhip_step", "lemhip_run", "lemhip_set_climate",
     "lemhip_copy_to_last_climate", "lemhip_summary", "lemhip_math_eval", "lemhip_device_info",
+    "lemhip_profile_enable", "lemhip_profile_read",
 ]
+
+PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
+              "mod2", "reductions", "step"]
 
 
 class LemError(RuntimeError):
@@ -116,6 +120,8 @@ def lib() -> C.CDLL:
     L.lemhip_summary.argtypes = [vp, C.POINTER(LemSummary)]
     L.lemhip_math_eval.argtypes = [C.c_int, C.c_int, vp, vp, vp]
     L.lemhip_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.lemhip_profile_enable.argtypes = [vp, C.c_int]
+    L.lemhip_profile_read.argtypes = [vp, C.c_int, vp, vp]
     _lib = L
     return L
 
@@ -223,6 +229,17 @@ class LemContext:
 
     def copy_to_last_climate(self) -> None:
         self._check(self.L.lemhip_copy_to_last_climate(self.h), "copy_to_last_climate")
+
+    def profile(self, on: bool) -> None:
+        """Start (and reset) or stop live HIP-event timing of the phases."""
+        self._check(self.L.lemhip_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_read(self) -> dict:
+        n = len(PROF_SLOTS)
+        ms = np.zeros(n, np.float64)
+        la = np.zeros(n, np.int64)
+        self._check(self.L.lemhip_profile_read(self.h, n, _ptr(ms), _ptr(la)), "profile_read")
+        return {k: (float(ms[i]), int(la[i])) for i, k in enumerate(PROF_SLOTS)}
 
     def summary(self) -> dict:
         s = LemSummary()
