@@ -23,11 +23,14 @@
  *           R(z) = sum_{k>=1} 2 z^k/(2k+1) (the atanh series), 11 terms.
  *  - log10: log(x) * (1/ln 10).
  *  - pow:   special cases, y == 2 -> x*x, base 10 -> exp of 10's exponent
- *           with a two-product split of z*ln10, else exp(y*log(x)).
+ *           with a two-product split of z*ln10; otherwise l = log(x) is
+ *           refined by c = (x - e^l)/e^l (x - e^l is exact, the two being
+ *           within a factor of two), y*l is carried as an exact two-product
+ *           p + err, and the result is e^p * (1 + err + y*c).
  *  - expm1: Taylor for |x| <= ln2/2, else 2^k(1+em1(r)) - 1.
  *  - tanh:  t = expm1(2|x|)/(expm1(2|x|)+2), sign restored; 1 for |x| > 22.
- * Accuracy against a correctly rounded libm is a few ulp (pow: up to ~10 ulp
- * for |y*log x| near 25); the unit tests state the bound they check.
+ * Accuracy against a correctly rounded libm is a few ulp; the unit tests
+ * state the bound they check.
  */
 #ifndef LEM_MATH_H
 #define LEM_MATH_H
@@ -185,7 +188,17 @@ LM_FN double lm_pow(double x, double y)
     if (x < 0.0) return lm_nan();
     if (x == lm_inf()) return y > 0.0 ? lm_inf() : 0.0;
     if (x == 10.0) return lm_exp10(y);
-    return lm_exp(y * lm_log(x));
+    double l = lm_log(x);
+    double e = lm_exp(l);
+    double c = (e > 0.0 && e < lm_inf()) ? (x - e) / e : 0.0;
+    double p = y * l;
+    double yh, yl, lh, ll;
+    lm_split(y, &yh, &yl);
+    lm_split(l, &lh, &ll);
+    double err = ((yh * lh - p) + yh * ll + yl * lh) + yl * ll; /* y*l - p, exactly */
+    double q = err + y * c;
+    double r = lm_exp(p);
+    return r + r * q;
 }
 
 LM_FN double lm_tanh(double x)
