@@ -114,6 +114,32 @@ int lemhip_copy_to_last_climate(lemhip_ctx *ctx);
  * mfd_simple.cu:354-355). */
 int lemhip_summary(lemhip_ctx *ctx, lem_summary *out);
 
+/* Atlas contexts: many DEMs of different sizes stepped together in one grid
+ * of atlas_w x atlas_h cells (no reference counterpart; the reference runs
+ * one DEM per process, lem.cpp:54-103).  Each DEM ("slot") occupies its own
+ * rectangle, padded by one nodata row and column, so its results are exactly
+ * those of a single-DEM context on the same inputs.  The per-slot calls mirror
+ * the single-DEM calls above, with host buffers in the DEM's own w*h layout
+ * and the outlet in the DEM's own row-major numbering.  Up to 255 slots.
+ * slot_open fails with LEMHIP_E_NOMEM when no rectangle is free; slot_close
+ * returns the rectangle.  step_slots takes one iteration number per slot
+ * (entries of unused slots are ignored).  The single-DEM calls
+ * init_state/upload/download/get_outlet/set_outlet/summary return
+ * LEMHIP_E_STATE on an atlas context. */
+int lemhip_create_atlas(const lem_params *p, int atlas_w, int atlas_h, int precision, int device_id,
+                        lemhip_ctx **out);
+int lemhip_slot_open(lemhip_ctx *ctx, int width, int height, int *slot);
+int lemhip_slot_init(lemhip_ctx *ctx, int slot, const double *dem, const double *rain, int soilT_mode);
+int lemhip_slot_close(lemhip_ctx *ctx, int slot);
+int lemhip_slot_rect(lemhip_ctx *ctx, int slot, int *x0, int *y0, int *width, int *height);
+int lemhip_slot_upload(lemhip_ctx *ctx, int slot, int field, const void *host);
+int lemhip_slot_download(lemhip_ctx *ctx, int slot, int field, void *host);
+int lemhip_slot_get_outlet(lemhip_ctx *ctx, int slot, int *outlet);
+int lemhip_slot_set_outlet(lemhip_ctx *ctx, int slot, int outlet);
+int lemhip_slot_set_climate(lemhip_ctx *ctx, int slot, double rain, double temp);
+int lemhip_slot_summary(lemhip_ctx *ctx, int slot, lem_summary *out);
+int lemhip_step_slots(lemhip_ctx *ctx, const int *iters);
+
 /* Live timing with HIP events on the context stream (no reference
  * counterpart; the reference logs phase times from cudaEvents at
  * routeflow.cu:955-971, mfd_simple.cu:376-380, mfd_accum.cu:474-480,

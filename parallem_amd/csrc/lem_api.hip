@@ -6,6 +6,7 @@
 #include <string.h>
 #include <new>
 #include <vector>
+#include <type_traits>
 
 static int grid1d(long n, int block = 256, int cap = 8192)
 {
@@ -17,28 +18,37 @@ static int grid1d(long n, int block = 256, int cap = 8192)
 
 // ------------------------------------------------------------ init kernels
 
-template <typename T>
-__global__ void k_init_state(int W, int H, lem_params p, const double *__restrict__ dem,
-                             const double *__restrict__ rain_in, int soilT_mode, uint8_t *mask, double *soilT,
-                             T *rain, T *temp, T *last_rain, T *last_temp, T *soilM, T *stone, T *fines, T *nut,
-                             T *soilB, T *TotB, T *w, T *slopeptr, T *ero, T *inci, T *geli, T *depo, T *dz, T *wC,
-                             T *wP, uint8_t *fd, uint8_t *SFD, T *slopes, long N)
+// Initial state of one DEM rectangle D of the grid (dem_in == nullptr: the
+// rectangle becomes nodata).  Inputs are in the DEM's own row-major layout.
+// memory.cu:207-208 (soil thickness), memory.cu:229-237 (mask),
+// Data.cpp:131-185 (process matrices), Data.cpp:231-271 (climate, first call)
+template <typename T, typename FS>
+__global__ void k_init_rect(int W, DemDesc D, int slot, lem_params p, const double *__restrict__ dem_in,
+                            const double *__restrict__ rain_in, int soilT_mode, uint8_t *demid, double *dem,
+                            uint8_t *mask, double *soilT, T *rain, T *temp, T *last_rain, T *last_temp, T *soilM,
+                            T *stone, T *fines, T *nut, T *soilB, T *TotB, T *w, T *slopeptr, T *ero, T *inci,
+                            T *geli, T *depo, T *dz, T *wC, T *wP, uint8_t *fd, uint8_t *SFD, T *slopes, FS *fa,
+                            long N)
 {
     double rainmax = p.rain * 2;                     // Data.cpp:246-250
     double rainmin = p.rain;
-    double increment = (rainmax - rainmin) / H;
+    double increment = (rainmax - rainmin) / D.h;
     double minheight = 467.228;
     double rescale = (10000 / (p.cellsize * p.cellsize));
-    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
-        int row = (int)(x / W);
-        double z = dem[x];
+    long n = (long)D.w * D.h;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        int row = (int)(k / D.w), col = (int)(k % D.w);
+        long x = (long)(D.y0 + row) * W + (D.x0 + col);
+        double z = dem_in ? dem_in[k] : -9999.0;
+        dem[x] = z;
+        if (demid) demid[x] = dem_in ? (uint8_t)slot : (uint8_t)DEM_NONE;
         bool m = !(z == -9999);                      // memory.cu:229-237
         mask[x] = m ? 1 : 0;
         double sT = (z > 0) ? ((z - 400) / 1400) * 5 : 0.0; // memory.cu:207-208
         if (soilT_mode == 1 && sT < 0.0) sT = 0.0;
         double r, t;
         if (m) {
-            r = rain_in ? rain_in[x] : rainmin + (row * increment);
+            r = rain_in ? rain_in[k] : rainmin + (row * increment);
             t = p.temp - ((z - minheight) * 0.0065);
         } else {
             r = -9999.;
@@ -58,43 +68,15 @@ __global__ void k_init_state(int W, int H, lem_params p, const double *__restric
             stone[x] = 0; fines[x] = 0; soilM[x] = 0; nut[x] = 0; soilB[x] = 0; TotB[x] = 0;
         }
         ero[x] = 0; geli[x] = 0; inci[x] = 0; depo[x] = 0; wC[x] = 0; wP[x] = 0;
+        fa[x] = FS{};
         for (int d = 0; d < 8; d++) slopes[d * N + x] = 0;   // Data.cpp:181-185
     }
 }
 
-// outlet = first strict minimum over mask cells, lowest starting at 1000 (Data.cpp:38-66)
-__global__ void k_outlet_first(long N, const double *dem, const uint8_t *mask, const unsigned long long *lowest_enc,
-                               int *idx)
+__global__ void k_slot_scal_init(DemScal *ds, int slot)
 {
-    double lowest = ord_dec(*lowest_enc);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
-        if (mask[i] && lowest < 1000.0 && dem[i] == lowest) atomicMin(idx, (int)i);
-}
-
-__global__ void k_outlet_set(LemScalars *sc, const int *idx, const unsigned long long *lowest_enc)
-{
-    double lowest = ord_dec(*lowest_enc);
-    sc->outlet = (lowest < 1000.0) ? *idx : 0;
-}
-
-__global__ void k_lowest_init(long N, const double *z, const uint8_t *mask, unsigned long long *out)
-{
-    __shared__ unsigned long long sh[256];
-    unsigned long long best = ord_enc(1000.0);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        double v = z[i];
-        if (mask[i] && v < 1000.0) {
-            unsigned long long e = ord_enc(v);
-            if (e < best) best = e;
-        }
-    }
-    sh[threadIdx.x] = best;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s && sh[threadIdx.x + s] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMin(out, sh[0]);
+    DemScal z = {};
+    ds[slot] = z;
 }
 
 // copytolastclimate (memory_dev.cu:363-373)
@@ -107,16 +89,17 @@ __global__ void k_copy_last(long N, const T *rain, const T *temp, T *last_rain, 
     }
 }
 
-// SetClimateMatrices(iteration >= 0) (Data.cpp:231-271)
+// SetClimateMatrices(iteration >= 0) (Data.cpp:231-271) on one DEM rectangle
 template <typename T>
-__global__ void k_set_climate(int W, int H, double rain_s, double temp_s, const double *dem, const uint8_t *mask,
+__global__ void k_set_climate(int W, DemDesc D, double rain_s, double temp_s, const double *dem, const uint8_t *mask,
                               T *rain, T *temp)
 {
-    long N = (long)W * H;
     double rainmax = rain_s * 2, rainmin = rain_s;
-    double increment = (rainmax - rainmin) / H;
-    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
-        int row = (int)(x / W);
+    double increment = (rainmax - rainmin) / D.h;
+    long n = (long)D.w * D.h;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        int row = (int)(k / D.w);
+        long x = (long)(D.y0 + row) * W + (D.x0 + (int)(k % D.w));
         if (mask[x]) {
             rain[x] = (T)(rainmin + (row * increment));
             temp[x] = (T)(temp_s - ((dem[x] - 467.228) * 0.0065));
@@ -127,6 +110,70 @@ __global__ void k_set_climate(int W, int H, double rain_s, double temp_s, const 
     }
 }
 
+__global__ void k_slots_first_iter(int ns, const int *flags, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ns) ds[j].first_iter = flags[j];
+}
+
+// ---------------------------------------------------------- atlas packing
+
+bool ShelfPack::alloc(int w, int h, int *x, int *y)
+{
+    int pw = w + 1, ph = h + 1;
+    int best = -1, bestk = -1, waste = 1 << 30;
+    for (int s = 0; s < (int)shelves.size(); s++) {
+        Shelf &S = shelves[s];
+        if (S.h < ph || S.h - ph >= waste) continue;
+        for (int k = 0; k < (int)S.free.size(); k++)
+            if (S.free[k].second - S.free[k].first >= pw) { best = s; bestk = k; waste = S.h - ph; break; }
+    }
+    if (best < 0) {
+        if (pw > W || top + ph > H) return false;
+        Shelf S;
+        S.y = top;
+        S.h = ph;
+        S.free.push_back({0, W});
+        shelves.push_back(S);
+        top += ph;
+        best = (int)shelves.size() - 1;
+        bestk = 0;
+    }
+    Shelf &S = shelves[best];
+    *x = S.free[bestk].first;
+    *y = S.y;
+    S.free[bestk].first += pw;
+    if (S.free[bestk].first == S.free[bestk].second) S.free.erase(S.free.begin() + bestk);
+    return true;
+}
+
+void ShelfPack::release(int x, int y, int w, int h)
+{
+    (void)h;
+    int pw = w + 1;
+    for (int s = 0; s < (int)shelves.size(); s++) {
+        Shelf &S = shelves[s];
+        if (S.y != y) continue;
+        auto &F = S.free;
+        size_t k = 0;
+        while (k < F.size() && F[k].first < x) k++;
+        F.insert(F.begin() + k, {x, x + pw});
+        for (size_t q = 0; q + 1 < F.size();) {
+            if (F[q].second == F[q + 1].first) { F[q].second = F[q + 1].second; F.erase(F.begin() + q + 1); }
+            else q++;
+        }
+        // drop empty shelves from the top so their rows can take a taller shelf
+        while (!shelves.empty()) {
+            Shelf &T = shelves.back();
+            if (T.free.size() == 1 && T.free[0].first == 0 && T.free[0].second == W) {
+                top = T.y;
+                shelves.pop_back();
+            } else break;
+        }
+        return;
+    }
+}
+
 // ------------------------------------------------------------- lifecycle
 
 static void free_all(lemhip_ctx *c)
@@ -134,11 +181,16 @@ static void free_all(lemhip_ctx *c)
     void *ptrs[] = {c->dem, c->soilT, c->mask, c->slopes, c->soilM, c->stone, c->fines, c->nut, c->soilB, c->TotB,
                     c->rain, c->temp, c->last_rain, c->last_temp, c->fd, c->SFD, c->w, c->fa, c->slopeptr, c->ero,
                     c->inci, c->geli, c->depo, c->dz, c->wC, c->wP, c->sp0, c->sp1, c->cand, c->ws, c->scan_tmp,
-                    c->z2, c->pl_front0, c->pl_front1, c->pl_U, c->pl_stamp, c->pl_Uv, c->pl_ctl, c->cub_tmp, c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
-                    c->dmask, c->tile_flags, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake, c->dag_ctl, c->partials, c->scal};
+                    c->z2, c->pl_front0, c->pl_front1, c->pl_U, c->pl_stamp, c->pl_Uv, c->pl_ctl, c->cub_tmp,
+                    c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
+                    c->dmask, c->tile_flags, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
+                    c->dag_ctl, c->partials, c->scal, c->demid, c->ddesc, c->dsc, c->slot_flags, c->labdem,
+                    c->dstage};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->hdsc) (void)hipHostFree(c->hdsc);
+    if (c->hflags) (void)hipHostFree(c->hflags);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->hint) (void)hipHostFree(c->hint);
     if (c->prof_ready)
@@ -150,23 +202,29 @@ static void free_all(lemhip_ctx *c)
     if (c->s) (void)hipStreamDestroy(c->s);
 }
 
-extern "C" int lemhip_create(const lem_params *p, int precision, int device_id, lemhip_ctx **out)
+static int upload_desc(lemhip_ctx *c)
 {
-    if (!p || !out) return LEMHIP_E_ARG;
-    *out = nullptr;
-    if (p->width <= 0 || p->height <= 0 || p->cellsize <= 0) return LEMHIP_E_ARG;
-    if (precision != LEMHIP_FP64 && precision != LEMHIP_FP32_KAHAN) return LEMHIP_E_ARG;
-    if ((long)p->width * p->height >= 0x7FFFFFFFL) return LEMHIP_E_ARG;  // labels and pointers are int32
-    lemhip_ctx *c = new (std::nothrow) lemhip_ctx();
-    if (!c) return LEMHIP_E_NOMEM;
+    int ns = 0;
+    for (int j = 0; j < LEM_MAX_SLOTS; j++)
+        if (c->hdesc[j].w > 0) ns = j + 1;
+    c->nslot = ns > 0 ? ns : 1;
+    LEM_CHECK(c, hipMemcpyAsync(c->ddesc, c->hdesc, sizeof(DemDesc) * (LEM_MAX_SLOTS + 1), hipMemcpyHostToDevice,
+                                c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    return LEMHIP_OK;
+}
+
+static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int precision, int device_id)
+{
     c->p = *p;
+    c->p.width = W;
+    c->p.height = H;
     c->prec = precision;
     c->dev = device_id;
-    c->W = p->width;
-    c->H = p->height;
-    c->N = (long)p->width * p->height;
+    c->W = W;
+    c->H = H;
+    c->N = (long)W * H;
     c->tsz = precision == LEMHIP_FP64 ? 8 : 4;
-    *out = c;
     if (hipSetDevice(device_id) != hipSuccess) { c->err = "hipSetDevice failed"; return LEMHIP_E_HIP; }
     LEM_CHECK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     long N = c->N;
@@ -201,19 +259,88 @@ extern "C" int lemhip_create(const lem_params *p, int precision, int device_id, 
     LEM_TRY(A((void **)&c->scan_tmp, N * 4));
     LEM_TRY(A((void **)&c->z2, N * 8));
     LEM_TRY(A((void **)&c->dmask, N));
+    LEM_TRY(A((void **)&c->labdem, N + 1));
     long ntiles = ((c->W + 63) / 64) * (long)((c->H + 15) / 16);
     LEM_TRY(A((void **)&c->tile_flags, ntiles * 4));
     for (int q = 0; q < 3; q++) LEM_TRY(A((void **)&c->dag_list[q], ntiles * 4));
     LEM_TRY(A((void **)&c->dag_wake, ntiles * 4));
     LEM_TRY(A(&c->dag_ctl, 256));
     LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));
-    LEM_TRY(A((void **)&c->partials, 4096 * 8));
+    LEM_TRY(A((void **)&c->partials, 64L * (LEM_MAX_SLOTS + 1) * 8));
     LEM_TRY(A((void **)&c->scal, sizeof(LemScalars)));
     LEM_CHECK(c, hipMemset(c->scal, 0, sizeof(LemScalars)));
+    LEM_TRY(A((void **)&c->ddesc, sizeof(DemDesc) * (LEM_MAX_SLOTS + 1)));
+    LEM_TRY(A((void **)&c->dsc, sizeof(DemScal) * (LEM_MAX_SLOTS + 1)));
+    LEM_CHECK(c, hipMemset(c->dsc, 0, sizeof(DemScal) * (LEM_MAX_SLOTS + 1)));
+    LEM_TRY(A((void **)&c->slot_flags, sizeof(int) * (LEM_MAX_SLOTS + 1)));
     LEM_CHECK(c, hipHostMalloc((void **)&c->hscal, sizeof(LemScalars), hipHostMallocDefault));
+    LEM_CHECK(c, hipHostMalloc((void **)&c->hdsc, sizeof(DemScal) * (LEM_MAX_SLOTS + 1), hipHostMallocDefault));
+    LEM_CHECK(c, hipHostMalloc((void **)&c->hflags, sizeof(int) * (LEM_MAX_SLOTS + 1), hipHostMallocDefault));
     LEM_CHECK(c, hipHostMalloc((void **)&c->hint, 64 * sizeof(int), hipHostMallocDefault));
     memset(c->hscal, 0, sizeof(LemScalars));
+    memset(c->hdsc, 0, sizeof(DemScal) * (LEM_MAX_SLOTS + 1));
+    memset(c->hflags, 0, sizeof(int) * (LEM_MAX_SLOTS + 1));
     return LEMHIP_OK;
+}
+
+static int check_create(const lem_params *p, int W, int H, int precision)
+{
+    if (!p || W <= 0 || H <= 0 || p->cellsize <= 0) return LEMHIP_E_ARG;
+    if (precision != LEMHIP_FP64 && precision != LEMHIP_FP32_KAHAN) return LEMHIP_E_ARG;
+    if ((long)W * H >= 0x7FFFFFFFL) return LEMHIP_E_ARG;  // labels and cell pointers are int32
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_create(const lem_params *p, int precision, int device_id, lemhip_ctx **out)
+{
+    if (!out) return LEMHIP_E_ARG;
+    *out = nullptr;
+    if (!p) return LEMHIP_E_ARG;
+    LEM_TRY(check_create(p, p->width, p->height, precision));
+    lemhip_ctx *c = new (std::nothrow) lemhip_ctx();
+    if (!c) return LEMHIP_E_NOMEM;
+    *out = c;
+    LEM_TRY(alloc_ctx(c, p, p->width, p->height, precision, device_id));
+    c->hrect[0] = DemDesc{0, 0, c->W, c->H};
+    c->hdesc[0] = c->hrect[0];
+    c->hstate[0] = 2;
+    return upload_desc(c);
+}
+
+extern "C" int lemhip_create_atlas(const lem_params *p, int atlas_w, int atlas_h, int precision, int device_id,
+                                   lemhip_ctx **out)
+{
+    if (!out) return LEMHIP_E_ARG;
+    *out = nullptr;
+    LEM_TRY(check_create(p, atlas_w, atlas_h, precision));
+    lemhip_ctx *c = new (std::nothrow) lemhip_ctx();
+    if (!c) return LEMHIP_E_NOMEM;
+    *out = c;
+    LEM_TRY(alloc_ctx(c, p, atlas_w, atlas_h, precision, device_id));
+    c->atlas = true;
+    c->pack.W = atlas_w;
+    c->pack.H = atlas_h;
+    LEM_CHECK(c, hipMalloc((void **)&c->demid, c->N));
+    LEM_CHECK(c, hipMemset(c->demid, DEM_NONE, c->N));
+    // every cell starts as nodata
+    DemDesc all{0, 0, c->W, c->H};
+    if (precision == LEMHIP_FP64)
+        k_init_rect<double, double><<<grid1d(c->N), 256, 0, c->s>>>(
+            c->W, all, DEM_NONE, c->p, nullptr, nullptr, 0, c->demid, c->dem, c->mask, c->soilT, (double *)c->rain,
+            (double *)c->temp, (double *)c->last_rain, (double *)c->last_temp, (double *)c->soilM, (double *)c->stone,
+            (double *)c->fines, (double *)c->nut, (double *)c->soilB, (double *)c->TotB, (double *)c->w,
+            (double *)c->slopeptr, (double *)c->ero, (double *)c->inci, (double *)c->geli, (double *)c->depo,
+            (double *)c->dz, (double *)c->wC, (double *)c->wP, c->fd, c->SFD, (double *)c->slopes, (double *)c->fa,
+            c->N);
+    else
+        k_init_rect<float, float2><<<grid1d(c->N), 256, 0, c->s>>>(
+            c->W, all, DEM_NONE, c->p, nullptr, nullptr, 0, c->demid, c->dem, c->mask, c->soilT, (float *)c->rain,
+            (float *)c->temp, (float *)c->last_rain, (float *)c->last_temp, (float *)c->soilM, (float *)c->stone,
+            (float *)c->fines, (float *)c->nut, (float *)c->soilB, (float *)c->TotB, (float *)c->w,
+            (float *)c->slopeptr, (float *)c->ero, (float *)c->inci, (float *)c->geli, (float *)c->depo,
+            (float *)c->dz, (float *)c->wC, (float *)c->wP, c->fd, c->SFD, (float *)c->slopes, (float2 *)c->fa, c->N);
+    LEM_LAUNCHED(c);
+    return upload_desc(c);
 }
 
 extern "C" int lemhip_destroy(lemhip_ctx *c)
@@ -239,33 +366,63 @@ static int stage_reserve(lemhip_ctx *c, size_t bytes)
     return LEMHIP_OK;
 }
 
-template <typename T>
-static int init_state_t(lemhip_ctx *c, const double *dem, const double *rain, int soilT_mode)
+static int dstage_reserve(lemhip_ctx *c, size_t bytes)
 {
-    long N = c->N;
-    LEM_CHECK(c, hipMemcpyAsync(c->dem, dem, N * 8, hipMemcpyHostToDevice, c->s));
-    double *rain_d = nullptr;
-    if (rain) {
-        rain_d = c->z2;
-        LEM_CHECK(c, hipMemcpyAsync(rain_d, rain, N * 8, hipMemcpyHostToDevice, c->s));
+    if (bytes <= c->dstage_bytes) return LEMHIP_OK;
+    if (c->dstage) (void)hipFree(c->dstage);
+    c->dstage = nullptr;
+    c->dstage_bytes = 0;
+    LEM_CHECK(c, hipMalloc(&c->dstage, bytes));
+    c->dstage_bytes = bytes;
+    return LEMHIP_OK;
+}
+
+static int slot_rect(lemhip_ctx *c, int slot, int need_state, DemDesc *D)
+{
+    if (slot < 0 || slot >= LEM_MAX_SLOTS || c->hstate[slot] < need_state) {
+        c->err = "slot not open";
+        return LEMHIP_E_ARG;
     }
-    k_init_state<T><<<grid1d(N), 256, 0, c->s>>>(
-        c->W, c->H, c->p, c->dem, rain_d, soilT_mode, c->mask, c->soilT, (T *)c->rain, (T *)c->temp,
-        (T *)c->last_rain, (T *)c->last_temp, (T *)c->soilM, (T *)c->stone, (T *)c->fines, (T *)c->nut,
-        (T *)c->soilB, (T *)c->TotB, (T *)c->w, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci, (T *)c->geli,
-        (T *)c->depo, (T *)c->dz, (T *)c->wC, (T *)c->wP, c->fd, c->SFD, (T *)c->slopes, N);
+    *D = c->hrect[slot];
+    return LEMHIP_OK;
+}
+
+template <typename T>
+static void launch_init_rect(lemhip_ctx *c, const DemDesc &D, int slot, const double *dem_d, const double *rain_d,
+                             int soilT_mode)
+{
+    using FS = typename std::conditional<sizeof(T) == 8, double, float2>::type;
+    long n = (long)D.w * D.h;
+    k_init_rect<T, FS><<<grid1d(n), 256, 0, c->s>>>(
+        c->W, D, slot, c->p, dem_d, rain_d, soilT_mode, c->demid, c->dem, c->mask, c->soilT, (T *)c->rain,
+        (T *)c->temp, (T *)c->last_rain, (T *)c->last_temp, (T *)c->soilM, (T *)c->stone, (T *)c->fines,
+        (T *)c->nut, (T *)c->soilB, (T *)c->TotB, (T *)c->w, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci,
+        (T *)c->geli, (T *)c->depo, (T *)c->dz, (T *)c->wC, (T *)c->wP, c->fd, c->SFD, (T *)c->slopes, (FS *)c->fa,
+        c->N);
+}
+
+// Data.cpp:38-66 (outlet), memory.cu:200-240 and Data.cpp:120-271 (initial
+// matrices) for one DEM rectangle
+static int init_slot(lemhip_ctx *c, int slot, const double *dem, const double *rain, int soilT_mode)
+{
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 1, &D));
+    long n = (long)D.w * D.h;
+    LEM_TRY(dstage_reserve(c, (rain ? 2 : 1) * n * 8));
+    double *dem_d = (double *)c->dstage;
+    double *rain_d = rain ? dem_d + n : nullptr;
+    LEM_CHECK(c, hipMemcpyAsync(dem_d, dem, n * 8, hipMemcpyHostToDevice, c->s));
+    if (rain) LEM_CHECK(c, hipMemcpyAsync(rain_d, rain, n * 8, hipMemcpyHostToDevice, c->s));
+    if (c->prec == LEMHIP_FP64) launch_init_rect<double>(c, D, slot, dem_d, rain_d, soilT_mode);
+    else launch_init_rect<float>(c, D, slot, dem_d, rain_d, soilT_mode);
     LEM_LAUNCHED(c);
-    LEM_CHECK(c, hipMemsetAsync(c->fa, 0, N * 8, c->s));
-    unsigned long long *low = reinterpret_cast<unsigned long long *>(c->partials);
-    int *idx = reinterpret_cast<int *>(c->partials + 1);
-    unsigned long long init = ord_enc(1000.0);
-    int big = 0x7FFFFFFF;
-    LEM_CHECK(c, hipMemcpyAsync(low, &init, 8, hipMemcpyHostToDevice, c->s));
-    LEM_CHECK(c, hipMemcpyAsync(idx, &big, 4, hipMemcpyHostToDevice, c->s));
-    k_lowest_init<<<grid1d(N, 256, 2048), 256, 0, c->s>>>(N, c->dem, c->mask, low);
-    k_outlet_first<<<grid1d(N), 256, 0, c->s>>>(N, c->dem, c->mask, low, idx);
-    k_outlet_set<<<1, 1, 0, c->s>>>(c->scal, idx, low);
+    k_slot_scal_init<<<1, 1, 0, c->s>>>(c->dsc, slot);
     LEM_LAUNCHED(c);
+    c->hdesc[slot] = D;
+    c->hstate[slot] = 2;
+    LEM_TRY(upload_desc(c));
+    LEM_TRY(lem_slot_lowest(c, c->dem, slot, 1));
+    LEM_TRY(lem_slot_first_lowest(c, c->dem, slot, 1));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     return LEMHIP_OK;
 }
@@ -273,9 +430,62 @@ static int init_state_t(lemhip_ctx *c, const double *dem, const double *rain, in
 extern "C" int lemhip_init_state(lemhip_ctx *c, const double *dem, const double *rain, int soilT_mode)
 {
     if (!c || !dem) return LEMHIP_E_ARG;
+    if (c->atlas) { c->err = "atlas context: use lemhip_slot_init"; return LEMHIP_E_STATE; }
     if (soilT_mode != 0 && soilT_mode != 1) { c->err = "soilT_mode must be 0 or 1"; return LEMHIP_E_ARG; }
-    return c->prec == LEMHIP_FP64 ? init_state_t<double>(c, dem, rain, soilT_mode)
-                                  : init_state_t<float>(c, dem, rain, soilT_mode);
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    return init_slot(c, 0, dem, rain, soilT_mode);
+}
+
+extern "C" int lemhip_slot_open(lemhip_ctx *c, int width, int height, int *slot)
+{
+    if (!c || !slot || width <= 0 || height <= 0) return LEMHIP_E_ARG;
+    *slot = -1;
+    if (!c->atlas) { c->err = "not an atlas context"; return LEMHIP_E_STATE; }
+    int j = 0;
+    while (j < LEM_MAX_SLOTS && c->hstate[j] != 0) j++;
+    if (j == LEM_MAX_SLOTS) { c->err = "all slots in use"; return LEMHIP_E_NOMEM; }
+    int x = 0, y = 0;
+    if (!c->pack.alloc(width, height, &x, &y)) { c->err = "no room in the atlas"; return LEMHIP_E_NOMEM; }
+    c->hrect[j] = DemDesc{x, y, width, height};
+    c->hstate[j] = 1;
+    *slot = j;
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_slot_init(lemhip_ctx *c, int slot, const double *dem, const double *rain, int soilT_mode)
+{
+    if (!c || !dem) return LEMHIP_E_ARG;
+    if (soilT_mode != 0 && soilT_mode != 1) { c->err = "soilT_mode must be 0 or 1"; return LEMHIP_E_ARG; }
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    return init_slot(c, slot, dem, rain, soilT_mode);
+}
+
+extern "C" int lemhip_slot_close(lemhip_ctx *c, int slot)
+{
+    if (!c) return LEMHIP_E_ARG;
+    if (!c->atlas) { c->err = "not an atlas context"; return LEMHIP_E_STATE; }
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 1, &D));
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    if (c->prec == LEMHIP_FP64) launch_init_rect<double>(c, D, DEM_NONE, nullptr, nullptr, 0);
+    else launch_init_rect<float>(c, D, DEM_NONE, nullptr, nullptr, 0);
+    LEM_LAUNCHED(c);
+    c->hdesc[slot] = DemDesc{0, 0, 0, 0};
+    c->hstate[slot] = 0;
+    c->pack.release(D.x0, D.y0, D.w, D.h);
+    return upload_desc(c);
+}
+
+extern "C" int lemhip_slot_rect(lemhip_ctx *c, int slot, int *x0, int *y0, int *w, int *h)
+{
+    if (!c) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 1, &D));
+    if (x0) *x0 = D.x0;
+    if (y0) *y0 = D.y0;
+    if (w) *w = D.w;
+    if (h) *h = D.h;
+    return LEMHIP_OK;
 }
 
 // ------------------------------------------------------- upload / download
@@ -314,80 +524,100 @@ static void *field_ptr(lemhip_ctx *c, int f, int *kind)
     }
 }
 
-extern "C" int lemhip_download(lemhip_ctx *c, int field, void *host)
+static size_t kind_elem(const lemhip_ctx *c, int kind)
 {
-    if (!c || !host) return LEMHIP_E_ARG;
+    switch (kind) {
+    case 0: case 4: return c->tsz;
+    case 1: case 5: return 8;
+    default: return 1;
+    }
+}
+
+// rectangle D of plane `plane` of a grid field <-> contiguous w*h block at `buf`
+static int rect_copy(lemhip_ctx *c, void *buf, void *base, int plane, size_t es, const DemDesc &D, bool down)
+{
+    char *dev = (char *)base + ((size_t)plane * c->N + (size_t)D.y0 * c->W + D.x0) * es;
+    char *blk = (char *)buf + (size_t)plane * D.w * D.h * es;
+    if (down)
+        LEM_CHECK(c, hipMemcpy2DAsync(blk, D.w * es, dev, c->W * es, D.w * es, D.h, hipMemcpyDeviceToHost, c->s));
+    else
+        LEM_CHECK(c, hipMemcpy2DAsync(dev, c->W * es, blk, D.w * es, D.w * es, D.h, hipMemcpyHostToDevice, c->s));
+    return LEMHIP_OK;
+}
+
+static int slot_download(lemhip_ctx *c, int slot, int field, void *host)
+{
     int kind = -1;
     void *src = field_ptr(c, field, &kind);
     if (!src) { c->err = "unknown field"; return LEMHIP_E_ARG; }
-    long N = c->N;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 2, &D));
     LEM_CHECK(c, hipSetDevice(c->dev));
-    if (kind == 1 || (kind == 0 && c->prec == LEMHIP_FP64) || (kind == 5 && c->prec == LEMHIP_FP64)) {
-        LEM_CHECK(c, hipMemcpyAsync(host, src, N * 8, hipMemcpyDeviceToHost, c->s));
-        LEM_CHECK(c, hipStreamSynchronize(c->s));
-        return LEMHIP_OK;
-    }
-    size_t bytes = (kind == 4) ? 8 * N * c->tsz : (kind == 5 ? N * 8 : (kind == 0 ? N * c->tsz : N));
+    long n = (long)D.w * D.h;
+    size_t es = kind_elem(c, kind);
+    int planes = kind == 4 ? 8 : 1;
+    size_t bytes = planes * n * es + (kind == 2 ? n : 0);
     LEM_TRY(stage_reserve(c, bytes));
-    LEM_CHECK(c, hipMemcpyAsync(c->stage, src, bytes, hipMemcpyDeviceToHost, c->s));
+    for (int q = 0; q < planes; q++) LEM_TRY(rect_copy(c, c->stage, src, q, es, D, true));
+    uint8_t *mstage = (uint8_t *)c->stage + planes * n * es;
+    if (kind == 2) LEM_TRY(rect_copy(c, mstage, c->mask, 0, 1, D, true));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
-    if (kind == 0) {
+    if (kind == 1 || (kind == 0 && c->prec == LEMHIP_FP64) || (kind == 5 && c->prec == LEMHIP_FP64)) {
+        memcpy(host, c->stage, n * 8);
+    } else if (kind == 0) {
         const float *s = (const float *)c->stage;
         double *d = (double *)host;
-        for (long i = 0; i < N; i++) d[i] = s[i];
+        for (long i = 0; i < n; i++) d[i] = s[i];
     } else if (kind == 5) {
         const float *s = (const float *)c->stage;
         double *d = (double *)host;
-        for (long i = 0; i < N; i++) d[i] = (double)s[2 * i] + (double)s[2 * i + 1];
-    } else if (kind == 2 || kind == 3) {
+        for (long i = 0; i < n; i++) d[i] = (double)s[2 * i] + (double)s[2 * i + 1];
+    } else if (kind == 3) {
         const uint8_t *s = (const uint8_t *)c->stage;
         int *d = (int *)host;
-        if (kind == 3) {
-            for (long i = 0; i < N; i++) d[i] = s[i];
-        } else {
-            // nodata cells carry -9999 in the reference's fd/SFD (newflow.cu:74-78)
-            std::vector<uint8_t> m(N);
-            LEM_CHECK(c, hipMemcpy(m.data(), c->mask, N, hipMemcpyDeviceToHost));
-            for (long i = 0; i < N; i++) d[i] = m[i] ? (int)s[i] : -9999;
-        }
+        for (long i = 0; i < n; i++) d[i] = s[i];
+    } else if (kind == 2) {
+        // nodata cells carry -9999 in the reference's fd/SFD (newflow.cu:74-78)
+        const uint8_t *s = (const uint8_t *)c->stage;
+        int *d = (int *)host;
+        for (long i = 0; i < n; i++) d[i] = mstage[i] ? (int)s[i] : -9999;
     } else if (kind == 4) {
         double *d = (double *)host;
         if (c->prec == LEMHIP_FP64) {
             const double *s = (const double *)c->stage;
             for (int k = 0; k < 8; k++)
-                for (long i = 0; i < N; i++) d[8 * i + k] = s[k * N + i];
+                for (long i = 0; i < n; i++) d[8 * i + k] = s[k * n + i];
         } else {
             const float *s = (const float *)c->stage;
             for (int k = 0; k < 8; k++)
-                for (long i = 0; i < N; i++) d[8 * i + k] = s[k * N + i];
+                for (long i = 0; i < n; i++) d[8 * i + k] = s[k * n + i];
         }
     }
     return LEMHIP_OK;
 }
 
-extern "C" int lemhip_upload(lemhip_ctx *c, int field, const void *host)
+static int slot_upload(lemhip_ctx *c, int slot, int field, const void *host)
 {
-    if (!c || !host) return LEMHIP_E_ARG;
     int kind = -1;
     void *dst = field_ptr(c, field, &kind);
     if (!dst) { c->err = "unknown field"; return LEMHIP_E_ARG; }
-    long N = c->N;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 2, &D));
     LEM_CHECK(c, hipSetDevice(c->dev));
+    long n = (long)D.w * D.h;
+    size_t es = kind_elem(c, kind);
+    int planes = kind == 4 ? 8 : 1;
+    LEM_TRY(stage_reserve(c, planes * n * es));
     if (kind == 1 || (kind == 0 && c->prec == LEMHIP_FP64) || (kind == 5 && c->prec == LEMHIP_FP64)) {
-        LEM_CHECK(c, hipMemcpyAsync(dst, host, N * 8, hipMemcpyHostToDevice, c->s));
-        LEM_CHECK(c, hipStreamSynchronize(c->s));
-        return LEMHIP_OK;
-    }
-    size_t bytes = (kind == 4) ? 8 * N * c->tsz : (kind == 5 ? N * 8 : (kind == 0 ? N * c->tsz : N));
-    LEM_TRY(stage_reserve(c, bytes));
-    if (kind == 0) {
+        memcpy(c->stage, host, n * 8);
+    } else if (kind == 0) {
         float *d = (float *)c->stage;
         const double *s = (const double *)host;
-        for (long i = 0; i < N; i++) d[i] = (float)s[i];
+        for (long i = 0; i < n; i++) d[i] = (float)s[i];
     } else if (kind == 5) {
         float *d = (float *)c->stage;
         const double *s = (const double *)host;
-        for (long i = 0; i < N; i++) {
+        for (long i = 0; i < n; i++) {
             float hi = (float)s[i];
             d[2 * i] = hi;
             d[2 * i + 1] = (float)(s[i] - (double)hi);
@@ -395,20 +625,70 @@ extern "C" int lemhip_upload(lemhip_ctx *c, int field, const void *host)
     } else if (kind == 2 || kind == 3) {
         uint8_t *d = (uint8_t *)c->stage;
         const int *s = (const int *)host;
-        for (long i = 0; i < N; i++) d[i] = (uint8_t)(s[i] < 0 ? 0 : s[i]);
+        for (long i = 0; i < n; i++) d[i] = (uint8_t)(s[i] < 0 ? 0 : s[i]);
     } else if (kind == 4) {
         const double *s = (const double *)host;
         if (c->prec == LEMHIP_FP64) {
             double *d = (double *)c->stage;
             for (int k = 0; k < 8; k++)
-                for (long i = 0; i < N; i++) d[k * N + i] = s[8 * i + k];
+                for (long i = 0; i < n; i++) d[k * n + i] = s[8 * i + k];
         } else {
             float *d = (float *)c->stage;
             for (int k = 0; k < 8; k++)
-                for (long i = 0; i < N; i++) d[k * N + i] = (float)s[8 * i + k];
+                for (long i = 0; i < n; i++) d[k * n + i] = (float)s[8 * i + k];
         }
     }
-    LEM_CHECK(c, hipMemcpyAsync(dst, c->stage, bytes, hipMemcpyHostToDevice, c->s));
+    for (int q = 0; q < planes; q++) LEM_TRY(rect_copy(c, c->stage, dst, q, es, D, false));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_download(lemhip_ctx *c, int field, void *host)
+{
+    if (!c || !host) return LEMHIP_E_ARG;
+    if (c->atlas) { c->err = "atlas context: use lemhip_slot_download"; return LEMHIP_E_STATE; }
+    return slot_download(c, 0, field, host);
+}
+
+extern "C" int lemhip_upload(lemhip_ctx *c, int field, const void *host)
+{
+    if (!c || !host) return LEMHIP_E_ARG;
+    if (c->atlas) { c->err = "atlas context: use lemhip_slot_upload"; return LEMHIP_E_STATE; }
+    return slot_upload(c, 0, field, host);
+}
+
+extern "C" int lemhip_slot_download(lemhip_ctx *c, int slot, int field, void *host)
+{
+    if (!c || !host) return LEMHIP_E_ARG;
+    return slot_download(c, slot, field, host);
+}
+
+extern "C" int lemhip_slot_upload(lemhip_ctx *c, int slot, int field, const void *host)
+{
+    if (!c || !host) return LEMHIP_E_ARG;
+    return slot_upload(c, slot, field, host);
+}
+
+// outlet cell index in the DEM's own row-major numbering
+extern "C" int lemhip_slot_get_outlet(lemhip_ctx *c, int slot, int *outlet)
+{
+    if (!c || !outlet) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 2, &D));
+    LEM_TRY(lem_read_slots(c));
+    int o = c->hdsc[slot].outlet;
+    *outlet = (o / c->W - D.y0) * D.w + (o % c->W - D.x0);
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_slot_set_outlet(lemhip_ctx *c, int slot, int outlet)
+{
+    if (!c) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 2, &D));
+    if (outlet < 0 || outlet >= D.w * D.h) return LEMHIP_E_ARG;
+    c->hint[8] = (D.y0 + outlet / D.w) * c->W + D.x0 + outlet % D.w;
+    LEM_CHECK(c, hipMemcpyAsync(&c->dsc[slot].outlet, &c->hint[8], sizeof(int), hipMemcpyHostToDevice, c->s));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     return LEMHIP_OK;
 }
@@ -416,22 +696,33 @@ extern "C" int lemhip_upload(lemhip_ctx *c, int field, const void *host)
 extern "C" int lemhip_get_outlet(lemhip_ctx *c, int *outlet)
 {
     if (!c || !outlet) return LEMHIP_E_ARG;
-    LEM_CHECK(c, hipMemcpyAsync(&c->hscal->outlet, &c->scal->outlet, sizeof(int), hipMemcpyDeviceToHost, c->s));
-    LEM_CHECK(c, hipStreamSynchronize(c->s));
-    *outlet = c->hscal->outlet;
-    return LEMHIP_OK;
+    if (c->atlas) { c->err = "atlas context: use lemhip_slot_get_outlet"; return LEMHIP_E_STATE; }
+    return lemhip_slot_get_outlet(c, 0, outlet);
 }
 
 extern "C" int lemhip_set_outlet(lemhip_ctx *c, int outlet)
 {
-    if (!c || outlet < 0 || outlet >= c->N) return LEMHIP_E_ARG;
-    c->hscal->outlet = outlet;
-    LEM_CHECK(c, hipMemcpyAsync(&c->scal->outlet, &c->hscal->outlet, sizeof(int), hipMemcpyHostToDevice, c->s));
-    LEM_CHECK(c, hipStreamSynchronize(c->s));
-    return LEMHIP_OK;
+    if (!c) return LEMHIP_E_ARG;
+    if (c->atlas) { c->err = "atlas context: use lemhip_slot_set_outlet"; return LEMHIP_E_STATE; }
+    return lemhip_slot_set_outlet(c, 0, outlet);
 }
 
 // --------------------------------------------------------------- phases
+
+// mfd_simple.cu:336: the outlet moves to the FA maximum on a DEM's first
+// iteration only
+static int set_first_iter(lemhip_ctx *c, const int *iters, int iter)
+{
+    int first = c->p.start_iter - c->p.runiniter;
+    for (int j = 0; j < c->nslot; j++) {
+        int it = iters ? iters[j] : iter;
+        c->hflags[j] = (c->hdesc[j].w > 0 && it == first) ? 1 : 0;
+    }
+    LEM_CHECK(c, hipMemcpyAsync(c->slot_flags, c->hflags, sizeof(int) * c->nslot, hipMemcpyHostToDevice, c->s));
+    k_slots_first_iter<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->slot_flags, c->dsc);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
 
 extern "C" int lemhip_flow_direction(lemhip_ctx *c, int iter)
 {
@@ -455,6 +746,7 @@ extern "C" int lemhip_flow_accum(lemhip_ctx *c, int iter)
 {
     if (!c) return LEMHIP_E_ARG;
     LEM_CHECK(c, hipSetDevice(c->dev));
+    LEM_TRY(set_first_iter(c, nullptr, iter));
     LEM_TRY(lem_fa_phase(c, iter));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     return LEMHIP_OK;
@@ -485,11 +777,11 @@ static int prof_resolve(lemhip_ctx *c)
     return LEMHIP_OK;
 }
 
-extern "C" int lemhip_step(lemhip_ctx *c, int iter)
+static int step_impl(lemhip_ctx *c, const int *iters, int iter)
 {
-    if (!c) return LEMHIP_E_ARG;
     LEM_CHECK(c, hipSetDevice(c->dev));
     prof_begin(c, PROF_STEP);
+    LEM_TRY(set_first_iter(c, iters, iter));
     LEM_TRY(lem_fd_phase(c, iter));
     LEM_TRY(lem_runoff_phase(c));
     LEM_TRY(lem_fa_phase(c, iter));
@@ -498,6 +790,18 @@ extern "C" int lemhip_step(lemhip_ctx *c, int iter)
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     LEM_TRY(prof_resolve(c));
     return LEMHIP_OK;
+}
+
+extern "C" int lemhip_step(lemhip_ctx *c, int iter)
+{
+    if (!c) return LEMHIP_E_ARG;
+    return step_impl(c, nullptr, iter);
+}
+
+extern "C" int lemhip_step_slots(lemhip_ctx *c, const int *iters)
+{
+    if (!c || !iters) return LEMHIP_E_ARG;
+    return step_impl(c, iters, 0);
 }
 
 extern "C" int lemhip_profile_enable(lemhip_ctx *c, int on)
@@ -543,17 +847,36 @@ extern "C" int lemhip_run(lemhip_ctx *c, int iter0, int nsteps)
     return LEMHIP_OK;
 }
 
+static int set_climate_rect(lemhip_ctx *c, const DemDesc &D, double rain, double temp)
+{
+    long n = (long)D.w * D.h;
+    if (c->prec == LEMHIP_FP64)
+        k_set_climate<double><<<grid1d(n), 256, 0, c->s>>>(c->W, D, rain, temp, c->dem, c->mask,
+                                                           (double *)c->rain, (double *)c->temp);
+    else
+        k_set_climate<float><<<grid1d(n), 256, 0, c->s>>>(c->W, D, rain, temp, c->dem, c->mask,
+                                                          (float *)c->rain, (float *)c->temp);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
 extern "C" int lemhip_set_climate(lemhip_ctx *c, double rain, double temp)
 {
     if (!c) return LEMHIP_E_ARG;
     LEM_CHECK(c, hipSetDevice(c->dev));
-    if (c->prec == LEMHIP_FP64)
-        k_set_climate<double><<<grid1d(c->N), 256, 0, c->s>>>(c->W, c->H, rain, temp, c->dem, c->mask,
-                                                              (double *)c->rain, (double *)c->temp);
-    else
-        k_set_climate<float><<<grid1d(c->N), 256, 0, c->s>>>(c->W, c->H, rain, temp, c->dem, c->mask,
-                                                             (float *)c->rain, (float *)c->temp);
-    LEM_LAUNCHED(c);
+    for (int j = 0; j < LEM_MAX_SLOTS; j++)
+        if (c->hdesc[j].w > 0) LEM_TRY(set_climate_rect(c, c->hdesc[j], rain, temp));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_slot_set_climate(lemhip_ctx *c, int slot, double rain, double temp)
+{
+    if (!c) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 2, &D));
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    LEM_TRY(set_climate_rect(c, D, rain, temp));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     return LEMHIP_OK;
 }
@@ -573,18 +896,20 @@ extern "C" int lemhip_copy_to_last_climate(lemhip_ctx *c)
     return LEMHIP_OK;
 }
 
-extern "C" int lemhip_summary(lemhip_ctx *c, lem_summary *o)
+extern "C" int lemhip_slot_summary(lemhip_ctx *c, int slot, lem_summary *o)
 {
     if (!c || !o) return LEMHIP_E_ARG;
-    LEM_CHECK(c, hipMemcpyAsync(c->hscal, c->scal, sizeof(LemScalars), hipMemcpyDeviceToHost, c->s));
-    LEM_CHECK(c, hipStreamSynchronize(c->s));
-    const LemScalars *h = c->hscal;
-    o->FA_max = h->FA_max;
-    o->totE = h->totE; o->totI = h->totI; o->totG = h->totG; o->totD = h->totD;
-    o->totweatherC = h->totwC; o->totweatherP = h->totwP; o->totbio = h->totbio;
-    o->outletcellidx = h->outlet;
-    o->sinkcount = c->h_sinkcount;
-    o->sinks_pass1 = c->h_sinks1;
+    DemDesc D;
+    LEM_TRY(slot_rect(c, slot, 2, &D));
+    LEM_TRY(lem_read_slots(c));
+    const DemScal &h = c->hdsc[slot];
+    o->FA_max = h.FA_max;
+    o->totE = h.totE; o->totI = h.totI; o->totG = h.totG; o->totD = h.totD;
+    o->totweatherC = h.totwC; o->totweatherP = h.totwP; o->totbio = h.totbio;
+    o->outletcellidx = (h.outlet / c->W - D.y0) * D.w + (h.outlet % c->W - D.x0);
+    o->sinkcount = h.sinkcount;
+    o->sinks_pass1 = h.sinks_pass1;
+    // the remaining counters describe the whole grid's last step
     o->nlabels = c->h_nlab;
     o->nedges = c->h_nedges;
     o->plateau_rounds1 = c->h_rounds1;
@@ -592,6 +917,13 @@ extern "C" int lemhip_summary(lemhip_ctx *c, lem_summary *o)
     o->fa_uncomputed = c->h_fa_unc;
     o->sed_uncomputed = c->h_sed_unc;
     return LEMHIP_OK;
+}
+
+extern "C" int lemhip_summary(lemhip_ctx *c, lem_summary *o)
+{
+    if (!c || !o) return LEMHIP_E_ARG;
+    if (c->atlas) { c->err = "atlas context: use lemhip_slot_summary"; return LEMHIP_E_STATE; }
+    return lemhip_slot_summary(c, 0, o);
 }
 
 // ------------------------------------------------------------ test hooks

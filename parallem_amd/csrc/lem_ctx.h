@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 #include "lem_device.h"
 #include "../../include/lemhip.h"
 
@@ -23,13 +24,43 @@ struct ProfSlot {
 // per-cell status of the dependency engine
 enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2 };
 
+// Shelf packing of DEM rectangles into the atlas grid.  Each request is
+// padded by one nodata column on the right and one nodata row below, so no
+// two DEMs are 8-neighbours (an off-grid neighbour and a nodata neighbour
+// behave the same on the whole path).
+struct ShelfPack {
+    struct Shelf {
+        int y, h;
+        std::vector<std::pair<int, int>> free;  // [x0, x1) intervals, sorted
+    };
+    int W = 0, H = 0, top = 0;
+    std::vector<Shelf> shelves;
+    bool alloc(int w, int h, int *x, int *y);
+    void release(int x, int y, int w, int h);
+};
+
 struct lemhip_ctx {
     lem_params p;
     int prec = LEMHIP_FP64;
     int dev = 0;
     hipStream_t s = nullptr;
-    int W = 0, H = 0;
+    int W = 0, H = 0;     // atlas grid (the DEM itself for a single-DEM context)
     long N = 0;
+    int nslot = 1;        // slots (DEMs) of the atlas
+    uint8_t *demid = nullptr;     // per-cell slot id (nullptr: one slot covering the grid)
+    DemDesc *ddesc = nullptr;     // device copy of hdesc
+    DemDesc hdesc[LEM_MAX_SLOTS + 1] = {};   // active rectangles (w == 0: inactive); entry 255 stays empty
+    DemDesc hrect[LEM_MAX_SLOTS] = {};       // reserved rectangles
+    uint8_t hstate[LEM_MAX_SLOTS] = {};      // 0 free, 1 reserved, 2 active
+    bool atlas = false;
+    ShelfPack pack;
+    DemScal *dsc = nullptr;       // device per-slot scalars (LEM_MAX_SLOTS + 1 entries)
+    DemScal *hdsc = nullptr;      // pinned host mirror
+    int *slot_flags = nullptr;    // device per-slot first-iteration flags
+    int *hflags = nullptr;        // pinned host mirror
+    uint8_t *labdem = nullptr;    // slot of each sink label
+    void *dstage = nullptr;       // device staging for slot initialisation
+    size_t dstage_bytes = 0;
     size_t tsz = 8;            // bytes of one bulk element (8 FP64, 4 FP32 form)
     std::string err;
 
@@ -146,4 +177,14 @@ int lem_runoff_phase(lemhip_ctx *c);
 int lem_fa_phase(lemhip_ctx *c, int iter);
 int lem_mod_phase(lemhip_ctx *c, int iter);
 int lem_sync_scalars(lemhip_ctx *c);
-int lem_sum_fields(lemhip_ctx *c, const void *const *fields, const int *is_f64, int nf, double *out_dev);
+// per-slot reductions (lem_mod.hip)
+int lem_slot_lowest(lemhip_ctx *c, const double *z, int s0, int ns);
+int lem_slot_first_lowest(lemhip_ctx *c, const double *z, int s0, int ns);
+int lem_read_slots(lemhip_ctx *c);
+static inline int lem_grid1d(long n, int block = 256, int cap = 8192)
+{
+    long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}

@@ -205,6 +205,7 @@ __global__ void k_fa_prep(int W, int H, const uint8_t *__restrict__ mask, const 
 // low byte sets E, W, NW, N, NE, so a nodata (or, D11, off-grid) neighbour at
 // E, SE, S, SW or W is a donor that never completes (D9)
 __global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ fd,
+                           const uint8_t *__restrict__ demid, const DemDesc *__restrict__ dd,
                            uint8_t *__restrict__ dm, int *__restrict__ st)
 {
     int icol = blockIdx.x * blockDim.x + threadIdx.x;
@@ -214,7 +215,9 @@ __global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const
     unsigned bits = 0;
     int s = 0;
     if (mask[i]) {
-        bool dead = (irow == 0 || icol == 0);
+        int lr = irow, lc = icol;
+        if (demid) { DemDesc D = dd[demid[i]]; lr -= D.y0; lc -= D.x0; }
+        bool dead = (lr == 0 || lc == 0);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             int d = kPULL[k];
@@ -520,7 +523,7 @@ int lem_sed_dag(lemhip_ctx *c)
 {
     dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 3) / 4);
     int *st = reinterpret_cast<int *>(c->scan_tmp);
-    k_sed_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->dmask, st);
+    k_sed_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->demid, c->ddesc, c->dmask, st);
     LEM_LAUNCHED(c);
     double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
     if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407

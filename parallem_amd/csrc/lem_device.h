@@ -37,6 +37,31 @@ __device__ __host__ __forceinline__ double ord_dec(uint64_t u)
     return lm_from_bits((u & 0x8000000000000000ULL) ? (u & 0x7FFFFFFFFFFFFFFFULL) : ~u);
 }
 
+// One DEM ("slot") of an atlas context: its rectangle in the atlas grid
+// (w == 0: slot unused).  Cells outside every rectangle are nodata.
+struct DemDesc {
+    int x0, y0, w, h;
+};
+#define DEM_NONE 255
+#define LEM_MAX_SLOTS 255
+
+// per-slot scalars (the reference's per-DEM host bookkeeping)
+struct DemScal {
+    double lowest;        // min over mask cells of dem, from 1000 (routeflow.cu:171-184)
+    double base_level;    // dem[outlet] - 0.00012 (updates.cu:151)
+    double FA_max;
+    double totE, totI, totG, totD, totwC, totwP, totbio;
+    unsigned long long lowest_enc, famax_enc;
+    int outlet;           // atlas index of the outlet cell
+    int argidx;
+    int sinks_pass1;
+    int sinkcount;
+    int zero_seen;
+    int first_iter;       // this step is the slot's first iteration (mfd_simple.cu:336)
+};
+
+__device__ __forceinline__ int slot_of(const uint8_t *demid, long i) { return demid ? (int)demid[i] : 0; }
+
 // per-step scalars kept on the device
 struct LemScalars {
     double lowest;        // min over mask cells of dem, starting at 1000 (routeflow.cu:171-184)

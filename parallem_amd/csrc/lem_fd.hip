@@ -4,6 +4,7 @@
 // surface, and the fd/SFD merge.
 #include "lem_ctx.h"
 #include <hipcub/hipcub.hpp>
+#include <stddef.h>
 
 // ------------------------------------------------------------------ helpers
 
@@ -63,36 +64,31 @@ __device__ __forceinline__ T block_reduce_sum(T v, T *sh)
 
 // ------------------------------------------------------------- A5: lowest
 
-// min over mask cells of dem with the reference's start value 1000 and strict
-// '<' (NaN never wins), as an order-preserving atomic min (exact, order-free)
-__global__ void k_lowest(long N, const double *__restrict__ z, const uint8_t *__restrict__ mask,
-                         unsigned long long *out)
+// routeflow.cu:739-741: the outlet is lowered below the lowest cell after the
+// directions were computed on the unadjusted surface (one thread per slot)
+__global__ void k_slots_outlet_adjust(double *dem, const DemDesc *dd, int ns, DemScal *ds)
 {
-    __shared__ unsigned long long sh[256];
-    unsigned long long best = ord_enc(1000.0);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        double v = z[i];
-        if (mask[i] && v < 1000.0) {
-            unsigned long long e = ord_enc(v);
-            if (e < best) best = e;
-        }
-    }
-    sh[threadIdx.x] = best;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s && sh[threadIdx.x + s] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMin(out, sh[0]);
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns || dd[j].w == 0) return;
+    double lowest = ord_dec(ds[j].lowest_enc);
+    ds[j].lowest = lowest;
+    if (lowest < dem[ds[j].outlet]) dem[ds[j].outlet] = lowest - 0.0001;
 }
 
-// routeflow.cu:739-741: the outlet is lowered below the lowest cell after the
-// directions were computed on the unadjusted surface
-__global__ void k_outlet_adjust(double *dem, LemScalars *sc)
+// routeflow.cu:953: fd[outlet] = WEST
+__global__ void k_slots_fd_outlet(uint8_t *fd, const DemDesc *dd, int ns, const DemScal *ds)
 {
-    double lowest = ord_dec(sc->lowest_enc);
-    sc->lowest = lowest;
-    if (lowest < dem[sc->outlet]) dem[sc->outlet] = lowest - 0.0001;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns || dd[j].w == 0) return;
+    fd[ds[j].outlet] = 16;
+}
+
+// per-slot counter adds from a block: LDS histogram over slots, then one atomic per slot touched
+__device__ __forceinline__ void slot_hist_flush(int *hist, DemScal *ds, int off, int ns)
+{
+    __syncthreads();
+    for (int j = threadIdx.x; j < ns; j += blockDim.x)
+        if (hist[j]) atomicAdd(reinterpret_cast<int *>(reinterpret_cast<char *>(&ds[j]) + off), hist[j]);
 }
 
 // ------------------------------------------------------------ A2: FlowDirs
@@ -259,39 +255,63 @@ __global__ void __launch_bounds__(256) k_plat_next(int W, int H, int r, const in
     }
 }
 
-__global__ void k_count_sinks(const int *cand, int n, const uint8_t *SFD, int *out)
+__global__ void k_slots_sinks_init(int ns, DemScal *ds)
 {
-    __shared__ int sh[256];
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = (k < n && SFD[cand[k]] == 0) ? 1 : 0;
-    v = block_reduce_sum<int>(v, sh);
-    if (threadIdx.x == 0 && v) atomicAdd(out, v);
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ns) ds[j].sinks_pass1 = 0;
+}
+
+__global__ void __launch_bounds__(256) k_count_sinks(const int *cand, int n, const uint8_t *SFD,
+                                                     const uint8_t *demid, int ns, DemScal *ds)
+{
+    __shared__ int hist[LEM_MAX_SLOTS + 1];
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        long i = cand[k];
+        if (SFD[i] == 0) atomicAdd(&hist[slot_of(demid, i)], 1);
+    }
+    slot_hist_flush(hist, ds, (int)offsetof(DemScal, sinks_pass1), ns);
 }
 
 // ------------------------------------------ A7-A9: watershed labels
 
-// sink flag: mask, not on the border rows/last column, SFD == 0 (newflow.cu:363-378)
-__global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *SFD, int *flag)
+// DEM-local border of init_watershedMFD (newflow.cu:371): last column, first
+// and last row of the cell's own DEM
+__device__ __forceinline__ bool dem_border(int W, int H, long i, const uint8_t *demid, const DemDesc *dd)
 {
-    long N = (long)W * H;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        int r = (int)(i / W), c = (int)(i % W);
-        bool border = (c == W - 1) || (r == 0) || (r == H - 1);
-        flag[i] = (mask[i] && !border && SFD[i] == 0) ? 1 : 0;
+    int r = (int)(i / W), c = (int)(i % W);
+    if (demid) {
+        DemDesc D = dd[demid[i]];
+        r -= D.y0; c -= D.x0;
+        return (c == D.w - 1) || (r == 0) || (r == D.h - 1);
     }
+    return (c == W - 1) || (r == 0) || (r == H - 1);
 }
 
-// init_watershedMFD with labels = 1 + row-major rank of the sink (D3)
-__global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD, const int *prefix, int *ws)
+// sink flag: mask, not on the DEM border, SFD == 0 (newflow.cu:363-378)
+__global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *SFD, const uint8_t *demid,
+                             const DemDesc *dd, int *flag)
+{
+    long N = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+        flag[i] = (mask[i] && SFD[i] == 0 && !dem_border(W, H, i, demid, dd)) ? 1 : 0;
+}
+
+// init_watershedMFD with labels = 1 + rank of the sink cell in atlas row-major order (D3)
+__global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD, const int *prefix,
+                          const uint8_t *demid, const DemDesc *dd, int *ws, uint8_t *labdem)
 {
     long N = (long)W * H;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         int r = (int)(i / W), c = (int)(i % W);
         int v;
         if (!mask[i]) v = 0;
-        else if ((c == W - 1) || (r == 0) || (r == H - 1)) v = 0;
-        else if (SFD[i] == 0) v = prefix[i] + 1;
-        else {
+        else if (dem_border(W, H, i, demid, dd)) v = 0;
+        else if (SFD[i] == 0) {
+            v = prefix[i] + 1;
+            labdem[v] = (uint8_t)slot_of(demid, i);
+        } else {
             int d = code_dir(SFD[i]);
             v = -((r + kDY[d]) * W + (c + kDX[d]) + 1);
         }
@@ -363,8 +383,9 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 // labels and z_c >= z_n contributes edge {ws_c, ws_n} at height z_c; the
 // table keeps the minimum per unordered pair (order-free, exact).
 __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__restrict__ mask, const int *__restrict__ ws,
-                                               const double *__restrict__ z, unsigned long long *keys,
-                                               unsigned long long *vals, long cap, int *overflow)
+                                               const double *__restrict__ z, const uint8_t *__restrict__ demid,
+                                               unsigned long long *keys, unsigned long long *vals, long cap,
+                                               int *overflow)
 {
     int icol = blockIdx.x * blockDim.x + threadIdx.x;
     int irow = blockIdx.y * blockDim.y + threadIdx.y;
@@ -379,6 +400,8 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
         int cx = icol + kDX[d], cy = irow + kDY[d];
         if (cx < 0 || cx >= W || cy < 0 || cy >= H) continue;
         long nb = (long)cy * W + cx;
+        // a neighbour outside the cell's own DEM is off-grid for that DEM
+        if (demid && demid[nb] != demid[c]) continue;
         int other = ws[nb];
         if (other == me) continue;
         if (!(zc >= z[nb])) continue;
@@ -403,7 +426,8 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
 
 __global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned long long *keys,
                                                        const unsigned long long *vals, uint32_t *elo, uint32_t *ehi,
-                                                       unsigned long long *ew, int *count, uint8_t *seen)
+                                                       unsigned long long *ew, int *count, uint8_t *seen,
+                                                       const uint8_t *labdem, DemScal *ds)
 {
     __shared__ int s_wave[4];
     __shared__ int s_base;
@@ -418,6 +442,7 @@ __global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned 
             ew[slot] = vals[i];
             seen[lo] = 1;
             seen[hi] = 1;
+            if (lo == 0) ds[labdem[hi]].zero_seen = 1;
         }
     }
 }
@@ -429,6 +454,36 @@ __global__ void k_count_bytes(long n, const uint8_t *a, int *out)
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) acc += a[i];
     acc = block_reduce_sum<int>(acc, sh);
     if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
+}
+
+// distinct labels of each slot in the edge set (newflow.cu:1045: data->sinkcount)
+__global__ void __launch_bounds__(256) k_count_labels(long n, const uint8_t *seen, const uint8_t *labdem, int ns,
+                                                      DemScal *ds)
+{
+    __shared__ int hist[LEM_MAX_SLOTS + 1];
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    for (long l = 1 + blockIdx.x * (long)blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x)
+        if (seen[l]) atomicAdd(&hist[labdem[l]], 1);
+    slot_hist_flush(hist, ds, (int)offsetof(DemScal, sinkcount), ns);
+}
+
+__global__ void k_slots_sinkcount_init(int ns, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    ds[j].sinkcount = 0;
+    ds[j].zero_seen = 0;
+}
+
+// label 0 counts as one of a slot's labels when the slot has an edge to it;
+// a slot without sinks skipped flooding and reports its sink count
+__global__ void k_slots_sinkcount_fin(const DemDesc *dd, int ns, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns || dd[j].w == 0) return;
+    if (ds[j].sinks_pass1 == 0) ds[j].sinkcount = 0;
+    else ds[j].sinkcount += ds[j].zero_seen;
 }
 
 __global__ void k_minimax_init(long n, unsigned long long *r, uint8_t *seen)
@@ -522,12 +577,13 @@ __global__ void k_fb_apply(long n, unsigned long long *r, const uint32_t *comp, 
 // raise_watershedMFD (newflow.cu:879-894) into a copy of the surface; label 0
 // and labels without a path keep raise = 0 as in the reference's zeroed array
 __global__ void k_raise(long N, const uint8_t *mask, const int *ws, const unsigned long long *r,
-                        const double *z, double *z2)
+                        const double *z, const uint8_t *demid, const DemScal *ds, double *z2)
 {
     const unsigned long long INF = ord_enc(lm_inf());
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         double v = z[i];
-        if (mask[i]) {
+        // floodingDriver only runs for a DEM with sinks left (routeflow.cu:913)
+        if (mask[i] && ds[slot_of(demid, i)].sinks_pass1 > 0) {
             int lab = ws[i];
             unsigned long long e = r[lab];
             double rs = (lab == 0 || e == INF) ? 0.0 : ord_dec(e);
@@ -538,14 +594,12 @@ __global__ void k_raise(long N, const uint8_t *mask, const int *ws, const unsign
 }
 
 // routeflow.cu:938-953
-__global__ void k_merge(long N, int flowtype, const uint8_t *mask, uint8_t *fd, const uint8_t *SFD, const LemScalars *sc)
+__global__ void k_merge(long N, int flowtype, const uint8_t *mask, uint8_t *fd, const uint8_t *SFD)
 {
-    int outlet = sc->outlet;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         uint8_t f = fd[i];
         if (flowtype == 0) f = SFD[i];
         else if (f == 0 && mask[i]) f = SFD[i];
-        if (i == outlet) f = 16;
         fd[i] = f;
     }
 }
@@ -721,7 +775,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     long N = c->N;
     int W = c->W, H = c->H;
     // labels
-    k_sink_flags<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, c->scan_tmp);
+    k_sink_flags<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, c->demid, c->ddesc, c->scan_tmp);
     LEM_LAUNCHED(c);
     size_t need = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->scan_tmp, c->ws, (int)N, c->s);
@@ -736,7 +790,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     LEM_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, need, c->scan_tmp, prefix, (int)N, c->s));
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[0], c->scan_tmp + (N - 1), sizeof(int), hipMemcpyDeviceToHost, c->s));
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[1], prefix + (N - 1), sizeof(int), hipMemcpyDeviceToHost, c->s));
-    k_init_ws<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, prefix, c->ws);
+    k_init_ws<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, prefix, c->demid, c->ddesc, c->ws, c->labdem);
     LEM_LAUNCHED(c);
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     int nlab = c->hint[0] + c->hint[1];
@@ -771,7 +825,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
         k_hash_clear<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals);
         LEM_CHECK(c, hipMemsetAsync(&c->scal->hash_overflow, 0, sizeof(int), c->s));
         dim3 blk(64, 4), grd((W + 63) / 64, (H + 3) / 4);
-        k_edges<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->hkeys, c->hvals, c->hcap,
+        k_edges<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->demid, c->hkeys, c->hvals, c->hcap,
                                        &c->scal->hash_overflow);
         LEM_LAUNCHED(c);
         LEM_CHECK(c, hipMemcpyAsync(&c->hscal->hash_overflow, &c->scal->hash_overflow, sizeof(int),
@@ -783,15 +837,15 @@ static int flooding(lemhip_ctx *c, int ncand1)
     }
     k_minimax_init<<<grid1d(nlab + 1), 256, 0, c->s>>>(nlab + 1, c->rmm, c->seen);
     LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
-    LEM_CHECK(c, hipMemsetAsync(&c->scal->sinkcount, 0, sizeof(int), c->s));
+    k_slots_sinkcount_init<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->dsc);
     k_edges_compact<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals, c->elo, c->ehi, c->ew,
-                                                       &c->scal->nedges, c->seen);
-    k_count_bytes<<<grid1d(nlab + 1, 256, 1024), 256, 0, c->s>>>(nlab + 1, c->seen, &c->scal->sinkcount);
+                                                       &c->scal->nedges, c->seen, c->labdem, c->dsc);
+    k_count_labels<<<grid1d(nlab + 1, 256, 1024), 256, 0, c->s>>>(nlab + 1, c->seen, c->labdem, c->nslot, c->dsc);
     LEM_LAUNCHED(c);
     LEM_TRY(read_scalars(c));
     int ne = c->hscal->nedges;
     c->h_nedges = ne;
-    c->h_sinkcount = c->hscal->sinkcount;
+
     for (int it = 0;; it += 16) {
         LEM_TRY(clear_flag(c));
         if (ne > 0)
@@ -810,7 +864,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     int cut = 0;
     LEM_TRY(read_flag(c, &cut));
     if (cut) LEM_TRY(cutoff_components(c, nlab, ne));
-    k_raise<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, c->rmm, c->dem, c->z2);
+    k_raise<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, c->rmm, c->dem, c->demid, c->dsc, c->z2);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
 }
@@ -822,37 +876,38 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
     long N = c->N;
     // lowest cell for the outlet check, taken on the unadjusted surface
     prof_begin(c, PROF_FD);
-    unsigned long long init = ord_enc(1000.0);
-    LEM_CHECK(c, hipMemcpyAsync(&c->scal->lowest_enc, &init, 8, hipMemcpyHostToDevice, c->s));
-    k_lowest<<<grid1d(N, 256, 2048), 256, 0, c->s>>>(N, c->dem, c->mask, &c->scal->lowest_enc);
-    LEM_LAUNCHED(c);
+    LEM_TRY(lem_slot_lowest(c, c->dem, 0, c->nslot));
     // pass 1 on the unraised surface; slopes persist (D7)
     int n1 = 0, r1 = 0;
     LEM_TRY(directions_and_plateaus<TS>(c, c->dem, reinterpret_cast<TS *>(c->slopes), &n1, &r1));
-    LEM_CHECK(c, hipMemsetAsync(&c->scal->sinks_pass1, 0, sizeof(int), c->s));
+    k_slots_sinks_init<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->dsc);
     if (n1 > 0) {
-        k_count_sinks<<<grid1d(n1, 256, 1 << 30), 256, 0, c->s>>>(c->cand, n1, c->SFD, &c->scal->sinks_pass1);
+        k_count_sinks<<<grid1d(n1, 256, 2048), 256, 0, c->s>>>(c->cand, n1, c->SFD, c->demid, c->nslot, c->dsc);
         LEM_LAUNCHED(c);
     }
-    LEM_CHECK(c, hipMemcpyAsync(&c->hscal->sinks_pass1, &c->scal->sinks_pass1, sizeof(int), hipMemcpyDeviceToHost, c->s));
-    LEM_CHECK(c, hipStreamSynchronize(c->s));
-    int sinks = c->hscal->sinks_pass1;
+    LEM_TRY(lem_read_slots(c));
+    int sinks = 0;
+    for (int j = 0; j < c->nslot; j++) sinks += c->hdesc[j].w > 0 ? c->hdsc[j].sinks_pass1 : 0;
+
     c->h_sinks1 = sinks;
     c->h_rounds1 = r1;
     c->h_rounds2 = 0;
     c->h_nlab = 0;
     c->h_nedges = 0;
     c->h_sinkcount = sinks;
+    if (sinks == 0) k_slots_sinkcount_init<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->dsc);
     if (sinks > 0) {
         prof_begin(c, PROF_FLOOD);
         LEM_TRY(flooding(c, n1));
+        k_slots_sinkcount_fin<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->ddesc, c->nslot, c->dsc);
         prof_end(c, PROF_FLOOD);
         int n2 = 0, r2 = 0;
         LEM_TRY(directions_and_plateaus<TS>(c, c->z2, (TS *)nullptr, &n2, &r2));
         c->h_rounds2 = r2;
     }
-    k_merge<<<grid1d(N), 256, 0, c->s>>>(N, c->p.flowdirectiontype, c->mask, c->fd, c->SFD, c->scal);
-    k_outlet_adjust<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    k_merge<<<grid1d(N), 256, 0, c->s>>>(N, c->p.flowdirectiontype, c->mask, c->fd, c->SFD);
+    k_slots_fd_outlet<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->fd, c->ddesc, c->nslot, c->dsc);
+    k_slots_outlet_adjust<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, c->nslot, c->dsc);
     LEM_LAUNCHED(c);
     prof_end(c, PROF_FD);
     return LEMHIP_OK;

@@ -3,6 +3,7 @@
 // cell in the reference's order, the sediment pass (lem_dag.hip) between
 // them, and deterministic reductions for the per-step totals.
 #include "lem_ctx.h"
+#include <stddef.h>
 
 template <typename T> int lem_fa_dag(lemhip_ctx *c);
 template <typename T> int lem_sed_dag(lemhip_ctx *c);
@@ -170,11 +171,12 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
                                               T *__restrict__ stone, T *__restrict__ fines, T *__restrict__ soilM,
                                               T *__restrict__ soilB, T *__restrict__ nut, T *__restrict__ TotB,
                                               double *__restrict__ dem, T *__restrict__ dzo, T *__restrict__ wCo,
-                                              T *__restrict__ wPo, const LemScalars *__restrict__ sc)
+                                              T *__restrict__ wPo, const uint8_t *__restrict__ demid,
+                                              const DemScal *__restrict__ ds)
 {
     const double rescale_w = 10000 * (10000 / (p.cellsize * p.cellsize));
     const double rescale_v = (10000 / (p.cellsize * p.cellsize));
-    const double base_level = sc->base_level;
+    const double base0 = ds[0].base_level;
     for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
         T tc = temp[self];
         if (mask[self] == 0) {
@@ -236,6 +238,7 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
         fines[self] = fi;
         soilM[self] = sm;
         // update surface (FP64 in every form)
+        const double base_level = demid ? ds[demid[self]].base_level : base0;
         double z = dem[self];
         z = z + (double)dz - (double)wC - (double)wP;
         if (z <= base_level) z = base_level + 0.01;
@@ -270,54 +273,146 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
     }
 }
 
-// updates.cu:151 and :165-174: base level from the outlet before the update,
-// then the outlet is pinned 0.00024 below its pre-update value
-__global__ void k_outlet_pre(const double *dem, LemScalars *sc) { sc->base_level = dem[sc->outlet] - 0.00012; }
-__global__ void k_outlet_post(double *dem, const LemScalars *sc)
+// ------------------------------------------------ per-slot reductions
+//
+// Grid (KB, ns): block (k, j) covers rows y0+k, y0+k+KB, ... of slot s0+j.
+// Sums go through per-block partials combined in a fixed order, so totals are
+// reproducible; min/max/first-index are exact under any order.
+
+#define SLOT_KB 64
+
+template <typename F>
+__device__ __forceinline__ void slot_rows(const DemDesc &D, int W, F f)
 {
-    dem[sc->outlet] = sc->base_level;
-    dem[sc->outlet] = dem[sc->outlet] - 0.00012;
-}
-
-// ------------------------------------------------ deterministic reductions
-
-#define RED_BLOCKS 512
-
-template <typename T>
-__global__ void k_sum_partial(long N, const T *__restrict__ a, double *__restrict__ partial)
-{
-    __shared__ double sh[256];
-    double acc = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
-        acc += (double)a[i];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
-}
-
-__global__ void k_sum_final(int nb, const double *__restrict__ partial, double *out)
-{
-    __shared__ double sh[256];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
+    for (int r = D.y0 + blockIdx.x; r < D.y0 + D.h; r += gridDim.x)
+        for (int c = D.x0 + threadIdx.x; c < D.x0 + D.w; c += blockDim.x) f((long)r * W + c);
 }
 
 template <typename T>
-static int dsum(lemhip_ctx *c, const void *a, double *out_dev)
+__global__ void __launch_bounds__(256) k_slot_sum(const T *__restrict__ a, int W, const DemDesc *dd, double *partial)
 {
-    k_sum_partial<T><<<RED_BLOCKS, 256, 0, c->s>>>(c->N, reinterpret_cast<const T *>(a), c->partials);
-    k_sum_final<<<1, 256, 0, c->s>>>(RED_BLOCKS, c->partials, out_dev);
+    __shared__ double sh[256];
+    DemDesc D = dd[blockIdx.y];
+    double acc = 0.0;
+    if (D.w > 0) slot_rows(D, W, [&](long i) { acc += (double)a[i]; });
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int t = 128; t > 0; t >>= 1) {
+        if (threadIdx.x < t) sh[threadIdx.x] += sh[threadIdx.x + t];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void k_slot_sum_final(const double *partial, int kb, int ns, DemScal *ds, int off)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    double acc = 0.0;
+    for (int k = 0; k < kb; k++) acc += partial[j * kb + k];
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(&ds[j]) + off) = acc;
+}
+
+template <typename T>
+static int slot_sum(lemhip_ctx *c, const void *a, int off)
+{
+    dim3 g(SLOT_KB, c->nslot);
+    k_slot_sum<T><<<g, 256, 0, c->s>>>(reinterpret_cast<const T *>(a), c->W, c->ddesc, c->partials);
+    k_slot_sum_final<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->partials, SLOT_KB, c->nslot, c->dsc, off);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+__device__ __forceinline__ unsigned long long block_min_u64(unsigned long long v, unsigned long long *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int t = 128; t > 0; t >>= 1) {
+        if (threadIdx.x < t && sh[threadIdx.x + t] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + t];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v, unsigned long long *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int t = 128; t > 0; t >>= 1) {
+        if (threadIdx.x < t && sh[threadIdx.x + t] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + t];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// lowest mask cell with the reference's start value 1000 and strict '<' (NaN never wins)
+__global__ void __launch_bounds__(256) k_slot_lowest(const double *__restrict__ z, const uint8_t *__restrict__ mask,
+                                                     int W, const DemDesc *dd, int s0, DemScal *ds)
+{
+    __shared__ unsigned long long sh[256];
+    int s = s0 + blockIdx.y;
+    DemDesc D = dd[s];
+    unsigned long long best = ord_enc(1000.0);
+    if (D.w > 0)
+        slot_rows(D, W, [&](long i) {
+            double v = z[i];
+            if (mask[i] && v < 1000.0) {
+                unsigned long long e = ord_enc(v);
+                if (e < best) best = e;
+            }
+        });
+    best = block_min_u64(best, sh);
+    if (threadIdx.x == 0 && D.w > 0) atomicMin(&ds[s].lowest_enc, best);
+}
+
+__global__ void __launch_bounds__(256) k_slot_first_lowest(const double *__restrict__ z,
+                                                           const uint8_t *__restrict__ mask, int W, const DemDesc *dd,
+                                                           int s0, DemScal *ds)
+{
+    int s = s0 + blockIdx.y;
+    DemDesc D = dd[s];
+    if (D.w == 0) return;
+    double lowest = ord_dec(ds[s].lowest_enc);
+    if (!(lowest < 1000.0)) return;
+    slot_rows(D, W, [&](long i) {
+        if (mask[i] && z[i] == lowest) atomicMin(&ds[s].argidx, (int)i);
+    });
+}
+
+__global__ void k_slots_lowest_init(int s0, int ns, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    ds[s0 + j].lowest_enc = ord_enc(1000.0);
+    ds[s0 + j].argidx = 0x7FFFFFFF;
+}
+
+// Data.cpp:38-66: first strict minimum, the DEM's cell 0 when none is below 1000
+__global__ void k_slots_outlet_from_lowest(int W, const DemDesc *dd, int s0, int ns, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    int s = s0 + j;
+    DemDesc D = dd[s];
+    if (D.w == 0) return;
+    double lowest = ord_dec(ds[s].lowest_enc);
+    ds[s].outlet = (lowest < 1000.0) ? ds[s].argidx : D.y0 * W + D.x0;
+}
+
+int lem_slot_lowest(lemhip_ctx *c, const double *z, int s0, int ns)
+{
+    k_slots_lowest_init<<<(ns + 255) / 256, 256, 0, c->s>>>(s0, ns, c->dsc);
+    dim3 g(SLOT_KB, ns);
+    k_slot_lowest<<<g, 256, 0, c->s>>>(z, c->mask, c->W, c->ddesc, s0, c->dsc);
+    LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+int lem_slot_first_lowest(lemhip_ctx *c, const double *z, int s0, int ns)
+{
+    dim3 g(SLOT_KB, ns);
+    k_slot_first_lowest<<<g, 256, 0, c->s>>>(z, c->mask, c->W, c->ddesc, s0, c->dsc);
+    k_slots_outlet_from_lowest<<<(ns + 255) / 256, 256, 0, c->s>>>(c->W, c->ddesc, s0, ns, c->dsc);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
 }
@@ -325,38 +420,77 @@ static int dsum(lemhip_ctx *c, const void *a, double *out_dev)
 __device__ __forceinline__ double fa_full(double v) { return v; }
 __device__ __forceinline__ double fa_full(float2 v) { return (double)v.x + (double)v.y; }
 
+// max of fa with the reference's start value 0 (NaN never wins)
 template <typename FS>
-__global__ void k_fa_max_full(long N, const FS *__restrict__ fa, unsigned long long *out)
+__global__ void __launch_bounds__(256) k_slot_famax(const FS *__restrict__ fa, int W, const DemDesc *dd, DemScal *ds)
 {
     __shared__ unsigned long long sh[256];
+    int s = blockIdx.y;
+    DemDesc D = dd[s];
     unsigned long long best = ord_enc(0.0);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        double v = fa_full(fa[i]);
-        if (v > ord_dec(best)) best = ord_enc(v);
-    }
-    sh[threadIdx.x] = best;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s && sh[threadIdx.x + s] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMax(out, sh[0]);
+    if (D.w > 0)
+        slot_rows(D, W, [&](long i) {
+            double v = fa_full(fa[i]);
+            if (v > ord_dec(best)) best = ord_enc(v);
+        });
+    best = block_max_u64(best, sh);
+    if (threadIdx.x == 0 && D.w > 0) atomicMax(&ds[s].famax_enc, best);
 }
 
 // first cell holding the maximum (mfd_simple.cu:339-349: strict '>' scan from 0)
 template <typename FS>
-__global__ void k_fa_argfirst(long N, const FS *__restrict__ fa, const unsigned long long *mx, int *idx)
+__global__ void __launch_bounds__(256) k_slot_fa_first(const FS *__restrict__ fa, int W, const DemDesc *dd, DemScal *ds)
 {
-    double m = ord_dec(*mx);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
-        if (m > 0.0 && fa_full(fa[i]) == m) atomicMin(idx, (int)i);
+    int s = blockIdx.y;
+    DemDesc D = dd[s];
+    if (D.w == 0 || !ds[s].first_iter) return;
+    double m = ord_dec(ds[s].famax_enc);
+    if (!(m > 0.0)) return;
+    slot_rows(D, W, [&](long i) {
+        if (fa_full(fa[i]) == m) atomicMin(&ds[s].argidx, (int)i);
+    });
 }
 
-__global__ void k_set_famax(LemScalars *sc, const unsigned long long *mx, const int *idx, int move_outlet)
+__global__ void k_slots_famax_init(int ns, DemScal *ds)
 {
-    double m = ord_dec(*mx);
-    sc->FA_max = m;
-    if (move_outlet) sc->outlet = (m > 0.0) ? *idx : 0;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    ds[j].famax_enc = ord_enc(0.0);
+    ds[j].argidx = 0x7FFFFFFF;
+}
+
+__global__ void k_slots_famax_set(int W, const DemDesc *dd, int ns, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns || dd[j].w == 0) return;
+    double m = ord_dec(ds[j].famax_enc);
+    ds[j].FA_max = m;
+    if (ds[j].first_iter) ds[j].outlet = (m > 0.0) ? ds[j].argidx : dd[j].y0 * W + dd[j].x0;
+}
+
+// updates.cu:151 and :165-174: base level from the outlet before the update,
+// then the outlet is pinned 0.00024 below its pre-update value
+__global__ void k_slots_base(const double *dem, const DemDesc *dd, int ns, DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns || dd[j].w == 0) return;
+    ds[j].base_level = dem[ds[j].outlet] - 0.00012;
+}
+
+__global__ void k_slots_post(double *dem, const DemDesc *dd, int ns, const DemScal *ds)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns || dd[j].w == 0) return;
+    int o = ds[j].outlet;
+    dem[o] = ds[j].base_level;
+    dem[o] = dem[o] - 0.00012;
+}
+
+int lem_read_slots(lemhip_ctx *c)
+{
+    LEM_CHECK(c, hipMemcpyAsync(c->hdsc, c->dsc, sizeof(DemScal) * c->nslot, hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    return LEMHIP_OK;
 }
 
 // ---------------------------------------------------------------- phases
@@ -378,40 +512,38 @@ int lem_runoff_phase(lemhip_ctx *c)
     return c->prec == LEMHIP_FP64 ? runoff_t<double>(c) : runoff_t<float>(c);
 }
 
+#define DS_OFF(f) ((int)offsetof(DemScal, f))
+
 template <typename T>
-static int fa_t(lemhip_ctx *c, int iter)
+static int fa_t(lemhip_ctx *c)
 {
     using FS = typename FaS<T>::type;
     prof_begin(c, PROF_FA);
     LEM_TRY(lem_fa_dag<T>(c));
     prof_end(c, PROF_FA, c->dag_rounds_last);
     // FA_max and the first-iteration outlet rule (mfd_simple.cu:332-357)
-    unsigned long long *mx = reinterpret_cast<unsigned long long *>(c->partials + RED_BLOCKS);
-    int *idx = reinterpret_cast<int *>(c->partials + RED_BLOCKS + 1);
-    unsigned long long z0 = ord_enc(0.0);
-    int big = 0x7FFFFFFF;
-    c->hint[4] = 0;
-    LEM_CHECK(c, hipMemcpyAsync(mx, &z0, 8, hipMemcpyHostToDevice, c->s));
-    LEM_CHECK(c, hipMemcpyAsync(idx, &big, 4, hipMemcpyHostToDevice, c->s));
-    k_fa_max_full<FS><<<RED_BLOCKS, 256, 0, c->s>>>(c->N, (const FS *)c->fa, mx);
-    int first = (iter == c->p.start_iter - c->p.runiniter);
-    if (first) k_fa_argfirst<FS><<<grid1d(c->N), 256, 0, c->s>>>(c->N, (const FS *)c->fa, mx, idx);
-    k_set_famax<<<1, 1, 0, c->s>>>(c->scal, mx, idx, first);
+    int ns = c->nslot;
+    dim3 g(SLOT_KB, ns);
+    k_slots_famax_init<<<(ns + 255) / 256, 256, 0, c->s>>>(ns, c->dsc);
+    k_slot_famax<FS><<<g, 256, 0, c->s>>>((const FS *)c->fa, c->W, c->ddesc, c->dsc);
+    k_slot_fa_first<FS><<<g, 256, 0, c->s>>>((const FS *)c->fa, c->W, c->ddesc, c->dsc);
+    k_slots_famax_set<<<(ns + 255) / 256, 256, 0, c->s>>>(c->W, c->ddesc, ns, c->dsc);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
 }
 
 int lem_fa_phase(lemhip_ctx *c, int iter)
 {
-    return c->prec == LEMHIP_FP64 ? fa_t<double>(c, iter) : fa_t<float>(c, iter);
+    (void)iter;
+    return c->prec == LEMHIP_FP64 ? fa_t<double>(c) : fa_t<float>(c);
 }
 
 template <typename T>
-static int mod_t(lemhip_ctx *c, int iter)
+static int mod_t(lemhip_ctx *c)
 {
-    (void)iter;
     using FS = typename FaS<T>::type;
     long N = c->N;
+    int ns = c->nslot;
     int g = grid1d(N, 256, 16384);
     prof_begin(c, PROF_MOD1);
     k_mod1<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa, (const T *)c->fines,
@@ -421,31 +553,32 @@ static int mod_t(lemhip_ctx *c, int iter)
     prof_end(c, PROF_MOD1);
     LEM_LAUNCHED(c);
     prof_begin(c, PROF_REDUCE);
-    LEM_TRY(dsum<T>(c, c->ero, &c->scal->totE));
-    LEM_TRY(dsum<T>(c, c->inci, &c->scal->totI));
-    LEM_TRY(dsum<T>(c, c->geli, &c->scal->totG));
+    LEM_TRY(slot_sum<T>(c, c->ero, DS_OFF(totE)));
+    LEM_TRY(slot_sum<T>(c, c->inci, DS_OFF(totI)));
+    LEM_TRY(slot_sum<T>(c, c->geli, DS_OFF(totG)));
     prof_end(c, PROF_REDUCE, 3);
     prof_begin(c, PROF_SED);
     LEM_TRY(lem_sed_dag<T>(c));
     prof_end(c, PROF_SED, c->dag_rounds_last);
-    LEM_TRY(dsum<T>(c, c->depo, &c->scal->totD));
-    k_outlet_pre<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    LEM_TRY(slot_sum<T>(c, c->depo, DS_OFF(totD)));
+    k_slots_base<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
     prof_begin(c, PROF_MOD2);
     k_mod2<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const FS *)c->fa, (const T *)c->ero, (const T *)c->inci,
                                    (const T *)c->geli, (const T *)c->depo, (const T *)c->temp, (const T *)c->rain,
                                    (const T *)c->last_temp, c->soilT, (T *)c->stone, (T *)c->fines, (T *)c->soilM,
                                    (T *)c->soilB, (T *)c->nut, (T *)c->TotB, c->dem, (T *)c->dz, (T *)c->wC,
-                                   (T *)c->wP, c->scal);
+                                   (T *)c->wP, c->demid, c->dsc);
     prof_end(c, PROF_MOD2);
-    k_outlet_post<<<1, 1, 0, c->s>>>(c->dem, c->scal);
+    k_slots_post<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
     LEM_LAUNCHED(c);
-    LEM_TRY(dsum<T>(c, c->wC, &c->scal->totwC));
-    LEM_TRY(dsum<T>(c, c->wP, &c->scal->totwP));
-    LEM_TRY(dsum<T>(c, c->TotB, &c->scal->totbio));
+    LEM_TRY(slot_sum<T>(c, c->wC, DS_OFF(totwC)));
+    LEM_TRY(slot_sum<T>(c, c->wP, DS_OFF(totwP)));
+    LEM_TRY(slot_sum<T>(c, c->TotB, DS_OFF(totbio)));
     return LEMHIP_OK;
 }
 
 int lem_mod_phase(lemhip_ctx *c, int iter)
 {
-    return c->prec == LEMHIP_FP64 ? mod_t<double>(c, iter) : mod_t<float>(c, iter);
+    (void)iter;
+    return c->prec == LEMHIP_FP64 ? mod_t<double>(c) : mod_t<float>(c);
 }

@@ -38,7 +38,10 @@ EXPORTS = [
     "lemhip_download", "lemhip_get_outlet", "lemhip_set_outlet", "lemhip_flow_direction", "lemhip_runoff",
     "lemhip_flow_accum", "lemhip_erosion", "lemhip_step", "lemhip_run", "lemhip_set_climate",
     "lemhip_copy_to_last_climate", "lemhip_summary", "lemhip_math_eval", "lemhip_device_info",
-    "lemhip_profile_enable", "lemhip_profile_read",
+    "lemhip_profile_enable", "lemhip_profile_read", "lemhip_create_atlas", "lemhip_slot_open",
+    "lemhip_slot_init", "lemhip_slot_close", "lemhip_slot_rect", "lemhip_slot_upload", "lemhip_slot_download",
+    "lemhip_slot_get_outlet", "lemhip_slot_set_outlet", "lemhip_slot_set_climate", "lemhip_slot_summary",
+    "lemhip_step_slots",
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
@@ -122,6 +125,19 @@ def lib() -> C.CDLL:
     L.lemhip_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.lemhip_profile_enable.argtypes = [vp, C.c_int]
     L.lemhip_profile_read.argtypes = [vp, C.c_int, vp, vp]
+    ip = C.POINTER(C.c_int)
+    L.lemhip_create_atlas.argtypes = [C.POINTER(LemParams), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.lemhip_slot_open.argtypes = [vp, C.c_int, C.c_int, ip]
+    L.lemhip_slot_init.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+    L.lemhip_slot_close.argtypes = [vp, C.c_int]
+    L.lemhip_slot_rect.argtypes = [vp, C.c_int, ip, ip, ip, ip]
+    L.lemhip_slot_upload.argtypes = [vp, C.c_int, C.c_int, vp]
+    L.lemhip_slot_download.argtypes = [vp, C.c_int, C.c_int, vp]
+    L.lemhip_slot_get_outlet.argtypes = [vp, C.c_int, ip]
+    L.lemhip_slot_set_outlet.argtypes = [vp, C.c_int, C.c_int]
+    L.lemhip_slot_set_climate.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+    L.lemhip_slot_summary.argtypes = [vp, C.c_int, C.POINTER(LemSummary)]
+    L.lemhip_step_slots.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -130,31 +146,30 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
-class LemContext:
-    """One DEM with its state resident on the GPU."""
+def _buf(name: str, n: int) -> np.ndarray:
+    if name == "Slopes":
+        return np.empty(8 * n, np.float64)
+    return np.empty(n, np.int32 if name in INT_FIELDS else np.float64)
 
-    def __init__(self, params: LemParams, precision: int = FP64, device: int = 0):
-        self.L = lib()
-        self.params = params
-        self.precision = precision
-        self.W, self.H = params.width, params.height
-        self.N = self.W * self.H
+
+class _Handle:
+    def _open(self, create, what: str) -> None:
         h = C.c_void_p()
-        rc = self.L.lemhip_create(C.byref(params), precision, device, C.byref(h))
+        rc = create(C.byref(h))
         self.h = h
         if rc != 0:
-            msg = self.L.lemhip_last_error(h).decode() if h.value else "lemhip_create failed"
+            msg = self.L.lemhip_last_error(h).decode() if h.value else f"{what} failed"
             if h.value:
                 self.L.lemhip_destroy(h)
             self.h = C.c_void_p()
-            raise LemError(f"lemhip_create: {rc} {msg}")
+            raise LemError(f"{what}: {rc} {msg}")
 
     def _check(self, rc: int, what: str) -> None:
         if rc != 0:
             raise LemError(f"{what}: {rc} {self.L.lemhip_last_error(self.h).decode()}")
 
     def close(self) -> None:
-        if self.h and self.h.value:
+        if getattr(self, "h", None) and self.h.value:
             self.L.lemhip_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -170,6 +185,32 @@ class LemContext:
     def __exit__(self, *a):
         self.close()
 
+    def profile(self, on: bool) -> None:
+        """Start (and reset) or stop live HIP-event timing of the phases."""
+        self._check(self.L.lemhip_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_read(self) -> dict:
+        n = len(PROF_SLOTS)
+        ms = np.zeros(n, np.float64)
+        la = np.zeros(n, np.int64)
+        self._check(self.L.lemhip_profile_read(self.h, n, _ptr(ms), _ptr(la)), "profile_read")
+        return {k: (float(ms[i]), int(la[i])) for i, k in enumerate(PROF_SLOTS)}
+
+    def copy_to_last_climate(self) -> None:
+        self._check(self.L.lemhip_copy_to_last_climate(self.h), "copy_to_last_climate")
+
+
+class LemContext(_Handle):
+    """One DEM with its state resident on the GPU."""
+
+    def __init__(self, params: LemParams, precision: int = FP64, device: int = 0):
+        self.L = lib()
+        self.params = params
+        self.precision = precision
+        self.W, self.H = params.width, params.height
+        self.N = self.W * self.H
+        self._open(lambda ph: self.L.lemhip_create(C.byref(params), precision, device, ph), "lemhip_create")
+
     # --- state -----------------------------------------------------------
     def init_state(self, dem: np.ndarray, rain: np.ndarray | None = None, soilT_mode: str = "formula") -> None:
         z = np.ascontiguousarray(dem, np.float64).reshape(-1)
@@ -180,12 +221,7 @@ class LemContext:
 
     def download(self, name: str) -> np.ndarray:
         f = FIELDS[name]
-        if name == "Slopes":
-            out = np.empty(8 * self.N, np.float64)
-        elif name in INT_FIELDS:
-            out = np.empty(self.N, np.int32)
-        else:
-            out = np.empty(self.N, np.float64)
+        out = _buf(name, self.N)
         self._check(self.L.lemhip_download(self.h, f, _ptr(out)), f"download {name}")
         return out
 
@@ -227,23 +263,98 @@ class LemContext:
     def set_climate(self, rain: float, temp: float) -> None:
         self._check(self.L.lemhip_set_climate(self.h, rain, temp), "set_climate")
 
-    def copy_to_last_climate(self) -> None:
-        self._check(self.L.lemhip_copy_to_last_climate(self.h), "copy_to_last_climate")
-
-    def profile(self, on: bool) -> None:
-        """Start (and reset) or stop live HIP-event timing of the phases."""
-        self._check(self.L.lemhip_profile_enable(self.h, 1 if on else 0), "profile_enable")
-
-    def profile_read(self) -> dict:
-        n = len(PROF_SLOTS)
-        ms = np.zeros(n, np.float64)
-        la = np.zeros(n, np.int64)
-        self._check(self.L.lemhip_profile_read(self.h, n, _ptr(ms), _ptr(la)), "profile_read")
-        return {k: (float(ms[i]), int(la[i])) for i, k in enumerate(PROF_SLOTS)}
-
     def summary(self) -> dict:
         s = LemSummary()
         self._check(self.L.lemhip_summary(self.h, C.byref(s)), "summary")
+        return s.as_dict()
+
+
+class LemAtlas(_Handle):
+    """Many DEMs of different sizes, each in its own rectangle ("slot") of one
+    device grid, stepped together.  Per-slot results equal those of a
+    LemContext on the same inputs."""
+
+    MAX_SLOTS = 255
+
+    def __init__(self, params: LemParams, width: int, height: int, precision: int = FP64, device: int = 0):
+        self.L = lib()
+        self.params = params
+        self.precision = precision
+        self.W, self.H = width, height
+        self.iters = np.zeros(self.MAX_SLOTS, np.int32)
+        self.shapes: dict[int, tuple[int, int]] = {}
+        self._open(lambda ph: self.L.lemhip_create_atlas(C.byref(params), width, height, precision, device, ph),
+                   "lemhip_create_atlas")
+
+    def open(self, width: int, height: int) -> int | None:
+        """Reserve a rectangle; None when the atlas has no room or no free slot."""
+        s = C.c_int(-1)
+        rc = self.L.lemhip_slot_open(self.h, width, height, C.byref(s))
+        if rc == -3:
+            return None
+        self._check(rc, "slot_open")
+        self.shapes[s.value] = (height, width)
+        return s.value
+
+    def init(self, slot: int, dem: np.ndarray, rain: np.ndarray | None = None, soilT_mode: str = "formula",
+             iteration: int = -1000) -> None:
+        h, w = self.shapes[slot]
+        z = np.ascontiguousarray(dem, np.float64).reshape(-1)
+        assert z.size == h * w
+        r = None if rain is None else np.ascontiguousarray(rain, np.float64).reshape(-1)
+        mode = {"formula": 0, "nonneg": 1}[soilT_mode]
+        self._check(self.L.lemhip_slot_init(self.h, slot, _ptr(z), None if r is None else _ptr(r), mode),
+                    "slot_init")
+        self.iters[slot] = iteration
+
+    def add(self, dem: np.ndarray, rain: np.ndarray | None = None, soilT_mode: str = "formula",
+            iteration: int = -1000) -> int | None:
+        dem = np.asarray(dem)
+        s = self.open(dem.shape[1], dem.shape[0])
+        if s is not None:
+            self.init(s, dem, rain, soilT_mode, iteration)
+        return s
+
+    def remove(self, slot: int) -> None:
+        self._check(self.L.lemhip_slot_close(self.h, slot), "slot_close")
+        self.shapes.pop(slot, None)
+
+    def rect(self, slot: int) -> tuple[int, int, int, int]:
+        v = [C.c_int() for _ in range(4)]
+        self._check(self.L.lemhip_slot_rect(self.h, slot, *[C.byref(x) for x in v]), "slot_rect")
+        return tuple(x.value for x in v)
+
+    def download(self, slot: int, name: str) -> np.ndarray:
+        h, w = self.shapes[slot]
+        out = _buf(name, h * w)
+        self._check(self.L.lemhip_slot_download(self.h, slot, FIELDS[name], _ptr(out)), f"download {name}")
+        return out
+
+    def upload(self, slot: int, name: str, a: np.ndarray) -> None:
+        dt = np.int32 if name in INT_FIELDS else np.float64
+        a = np.ascontiguousarray(a, dt).reshape(-1)
+        self._check(self.L.lemhip_slot_upload(self.h, slot, FIELDS[name], _ptr(a)), f"upload {name}")
+
+    def outlet(self, slot: int) -> int:
+        o = C.c_int()
+        self._check(self.L.lemhip_slot_get_outlet(self.h, slot, C.byref(o)), "slot_get_outlet")
+        return o.value
+
+    def set_outlet(self, slot: int, v: int) -> None:
+        self._check(self.L.lemhip_slot_set_outlet(self.h, slot, int(v)), "slot_set_outlet")
+
+    def set_climate(self, slot: int, rain: float, temp: float) -> None:
+        self._check(self.L.lemhip_slot_set_climate(self.h, slot, rain, temp), "slot_set_climate")
+
+    def step(self) -> None:
+        """One timestep of every active slot at its own iteration number."""
+        self._check(self.L.lemhip_step_slots(self.h, _ptr(self.iters)), "step_slots")
+        for s in self.shapes:
+            self.iters[s] += 1
+
+    def summary(self, slot: int) -> dict:
+        s = LemSummary()
+        self._check(self.L.lemhip_slot_summary(self.h, slot, C.byref(s)), "slot_summary")
         return s.as_dict()
 
 
