@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark: cells/s per landscape-evolution timestep at 4096^2 on one MI355X.
+"""Benchmark: cells/s per landscape-evolution timestep on 4096^2 DEMs, one MI355X.
 
-Workload (the configuration the metric is quoted on): one 4096 x 4096 fBm DEM
-(diamond-square, H = 0.8, 0-800 m, 0.5 % tilt, 400 pits, 100 plateaus, 5 %
-nodata frame), 30 m cells, per-cell lognormal(0, 0.3) rainfall rescaled to a
-1000 mm/yr mean, SFD routing, default parameters.  Soil thickness starts from
-the reference formula clamped at 0 (the unclamped formula turns every cell
-below 400 m into NaN after one step).
+Workloads (all with state resident in HBM, 30 m cells, SFD routing, default
+parameters, soil thickness from the reference formula clamped at 0 — the
+unclamped formula turns every cell below 400 m into NaN after one step):
 
-A step is one full FD -> FA -> MOD timestep of the whole DEM with all state
-resident in HBM.  `value` is the FP32 + compensated-accumulation form; the
-plain FP64 form (bit-exact with the CPU restatement) is timed beside it.
-Each step call returns after a stream synchronisation, so the wall clock
-around the K timed steps brackets the device work exactly.
+* value — batched FP32 + compensated accumulation: B (default 4) independent
+  4096 x 4096 fBm DEMs (diamond-square H = 0.8, 0-800 m, 0.5 % tilt, 400 pits,
+  100 plateaus, 5 % nodata frame, per-cell lognormal(0, 0.3) rain rescaled to
+  a 1000 mm/yr mean; seeds differ per DEM) packed in one atlas grid and
+  stepped together;
+* single_fp32 — one such DEM alone, same precision form;
+* plain_fp64 — one such DEM alone in the plain FP64 form (bit-exact with the
+  CPU restatement): the reference's own arithmetic.
+
+A step is one full FD -> FA -> MOD timestep of every DEM.  Every step call
+returns after a stream synchronisation, so the wall clock around the K timed
+steps brackets the device work exactly.  cells counted = sum of W*H over DEMs.
 """
 import argparse
 import json
@@ -28,55 +32,122 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cells/s per LEM timestep at 4096² (batched FP32 vs plain); p50/p99 step latency"
 HBM_PEAK_GBS = 8000.0
+SIDE = 4096
 
-# Algorithmic bytes per cell-timestep (each persistent field read and written
-# once, each hand-off written once and read once), with this build's layouts:
+# Algorithmic bytes per cell-timestep of the whole step (each persistent field
+# read and written once, each hand-off written once and read once), with this
+# build's layouts (u8 fd instead of the reference's i32):
 #   FP64 form: dem 8/8, Slopes 8x8/8x8, soilT 8/8, 6 soil/veg fields 48/48,
-#              4 climate fields 32/0, fd 1/1 (u8), fa 8/8            -> 306 B
+#              4 climate fields 32/0, fd 1/1, fa 8/8                   -> 306 B
 #   FP32 form: dem 8/8, Slopes 8x4/8x4, soilT 8/8, 6 fields 24/24,
-#              4 climate fields 16/0, fd 1/1, fa 8/8 (float pair)     -> 178 B
+#              4 climate fields 16/0, fd 1/1, fa 8/8 (float pair)       -> 178 B
 B_ALG = {"fp64": 306, "fp32": 178}
 
-# Algorithmic bytes per cell of the flow-accumulation pass (the phase the
-# timing identifies as dominant): donor bits built from fd (1 B), runoff
-# weight read (T), fa written once and read once by its receivers (8 + 8 B).
-B_FA = {"fp64": 1 + 8 + 16, "fp32": 1 + 4 + 16}
-# MOD part 2 kernel (weathering + surface/nutrient/vegetation update), per
-# cell: reads fa, ero, inci, geli, depo, temp, rain, last_temp, soilT(8),
-# stone, fines, soilM, soilB, nut, TotB, dem(8); writes dz, wC, wP, soilT(8),
-# stone, fines, soilM, soilB, nut, TotB, dem(8).
-def b_mod2(t):
-    fa = 8
-    return fa + 7 * t + 8 + 6 * t + 8 + 3 * t + 8 + 6 * t + 8
+
+def kernel_bytes(kernel, t):
+    """Algorithmic HBM bytes per cell for one pass of a kernel family over the
+    grid (t = bytes of a bulk FP element: 8 FP64 form, 4 FP32 form).
+
+    fa_rounds  (k_dag_round, flow accumulation): donor bits 1 + fd 1 + runoff
+               weight t read, fa 8 written (f64, or float pair).
+    sed_rounds (k_dag_round, sediment routing): donor bits 1 + fd 1, ero, inci,
+               geli, stone t each + soilT 8 read; depo, stone, fines t each +
+               soilT 8 written.
+    mod2       (k_mod2): fa 8, ero, inci, geli, depo, temp, rain, last_temp,
+               stone, fines, soilM, soilB, nut, TotB t each, soilT 8, dem 8
+               read; dz, wC, wP, stone, fines, soilM, soilB, nut, TotB t each,
+               soilT 8, dem 8 written.
+    mod1       (k_mod1): Slopes 8t, fa 8, fines, nut, stone, TotB, temp t each,
+               soilT 8 read; SlopePtr, ero, inci, geli, depo t each written.
+    flowdirs   (k_flowdirs): dem 8 (stencil, read once from HBM), mask 1 read;
+               fd, SFD 1 each + downhill Slopes entries (at most 8t) written.
+    """
+    return {
+        "fa_rounds": 2 + t + 8,
+        "sed_rounds": 2 + 4 * t + 8 + 3 * t + 8,
+        "mod2": 8 + 13 * t + 8 + 8 + 9 * t + 8 + 8,
+        "mod1": 8 * t + 8 + 5 * t + 8 + 5 * t,
+        "flowdirs": 8 + 1 + 2 + 8 * t,
+    }.get(kernel)
 
 
-def gen_dem(side):
+# kernel-level timing slots (HIP events around exactly these launches)
+KERNEL_SLOTS = ("fa_rounds", "sed_rounds", "mod1", "mod2", "flowdirs")
+# rocprofv3 kernel names of those slots
+ROCPROF_NAMES = {"fa_rounds": "k_dag_round<64, 16, FaCell", "sed_rounds": "k_dag_round<64, 16, SedCell",
+                 "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs"}
+
+
+def dems(count):
     from parallem_amd import synth
-    return synth.config_dem(2, side=side)
+    return [synth.config_dem(2, seed_offset=k, side=SIDE) for k in range(count)]
 
 
-def run_form(side, dem, precision, steps, warmup, profile):
-    from parallem_amd import lemhip
-    ctx = lemhip.LemContext(lemhip.default_params(side, side, flowtype=0), precision, 0)
-    ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
-    it = -1000
+def timed(step, steps, warmup, handle):
     for _ in range(warmup):
-        ctx.step(it)
-        it += 1
-    if profile:
-        ctx.profile(True)
+        step()
+    handle.profile(True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
         a = time.perf_counter()
-        ctx.step(it)
+        step()
         lat.append(time.perf_counter() - a)
-        it += 1
     total = time.perf_counter() - t0
-    prof = ctx.profile_read() if profile else None
+    return total, np.array(lat), handle.profile_read()
+
+
+def run_single(dem, precision, steps, warmup):
+    from parallem_amd import lemhip
+    ctx = lemhip.LemContext(lemhip.default_params(SIDE, SIDE, flowtype=0), precision, 0)
+    ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+    it = [-1000]
+
+    def step():
+        ctx.step(it[0])
+        it[0] += 1
+    total, lat, prof = timed(step, steps, warmup, ctx)
     summ = ctx.summary()
     ctx.close()
-    return total, np.array(lat), prof, summ
+    return total, lat, prof, summ
+
+
+def run_batched(ds, precision, steps, warmup):
+    from parallem_amd import lemhip
+    cols = int(np.ceil(np.sqrt(len(ds))))
+    rows = int(np.ceil(len(ds) / cols))
+    aw, ah = cols * (SIDE + 1), rows * (SIDE + 1)
+    atlas = lemhip.LemAtlas(lemhip.default_params(aw, ah, flowtype=0), aw, ah, precision, 0)
+    slots = [atlas.add(d.z, d.rain, soilT_mode="nonneg") for d in ds]
+    assert all(s is not None for s in slots)
+    total, lat, prof = timed(atlas.step, steps, warmup, atlas)
+    summ = [atlas.summary(s) for s in slots]
+    atlas.close()
+    return total, lat, prof, summ, (aw, ah)
+
+
+def phase_ms(prof, steps):
+    return {k: round(v[0] / steps, 3) for k, v in prof.items()}
+
+
+def roofline(prof, steps, cells, t):
+    """Dominant kernel by live HIP-event time; achieved = algorithmic bytes
+    per launch / average launch duration (= bytes of all its launches / their
+    total time)."""
+    dom = max(KERNEL_SLOTS, key=lambda k: prof[k][0])
+    ms, launches = prof[dom]
+    b = kernel_bytes(dom, t)
+    # the round kernels share one pass over the grid per step; the others make
+    # one pass per launch
+    passes = steps if dom in ("fa_rounds", "sed_rounds") else launches
+    total_bytes = b * cells * passes
+    ach = total_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+    return {"kernel": ROCPROF_NAMES[dom], "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": ach, "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None,
+            "bytes_per_cell": b, "launches_per_step": launches / steps,
+            "avg_launch_us": 1e3 * ms / launches if launches else None,
+            "bytes_per_launch": total_bytes / launches if launches else None,
+            "ms_per_step": ms / steps}
 
 
 def cpu_baseline(side, threads):
@@ -96,8 +167,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--side", type=int, default=4096)
-    ap.add_argument("--no-plain", action="store_true", help="skip the FP64 form")
+    ap.add_argument("--batch", type=int, default=4, help="4096^2 DEMs in the batched form")
+    ap.add_argument("--no-plain", action="store_true", help="skip the single-DEM forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-side", type=int, default=2048)
     args = ap.parse_args()
@@ -107,43 +178,30 @@ def main():
         sys.exit(2)
 
     from parallem_amd import lemhip
-    side = args.side
-    N = side * side
-    print(f"[bench] generating {side}x{side} DEM", file=sys.stderr, flush=True)
-    dem = gen_dem(side)
+    print(f"[bench] generating {args.batch} DEMs of {SIDE}x{SIDE}", file=sys.stderr, flush=True)
+    ds = dems(max(args.batch, 1))
+    cells_b = args.batch * SIDE * SIDE
 
-    print("[bench] FP32+Kahan form", file=sys.stderr, flush=True)
-    t32, lat32, prof32, sum32 = run_form(side, dem, lemhip.FP32_KAHAN, args.steps, args.warmup, True)
-    value = N * args.steps / t32
-    plain = None
+    print("[bench] batched FP32+Kahan form", file=sys.stderr, flush=True)
+    tb, latb, profb, summb, atlas_dims = run_batched(ds[:args.batch], lemhip.FP32_KAHAN, args.steps, args.warmup)
+    value = cells_b * args.steps / tb
+
+    single32 = plain = None
     if not args.no_plain:
-        print("[bench] plain FP64 form", file=sys.stderr, flush=True)
-        t64, lat64, prof64, _ = run_form(side, dem, lemhip.FP64, args.steps, args.warmup, True)
-        plain = {"value": N * args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps,
+        print("[bench] single-DEM FP32+Kahan form", file=sys.stderr, flush=True)
+        t32, lat32, prof32, s32 = run_single(ds[0], lemhip.FP32_KAHAN, args.steps, args.warmup)
+        single32 = {"value": SIDE * SIDE * args.steps / t32, "ms_per_step": 1e3 * t32 / args.steps,
+                    "p50_ms": float(np.percentile(lat32, 50) * 1e3), "p99_ms": float(np.percentile(lat32, 99) * 1e3),
+                    "phase_ms_per_step": phase_ms(prof32, args.steps)}
+        print("[bench] single-DEM plain FP64 form", file=sys.stderr, flush=True)
+        t64, lat64, prof64, s64 = run_single(ds[0], lemhip.FP64, args.steps, args.warmup)
+        plain = {"value": SIDE * SIDE * args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps,
                  "p50_ms": float(np.percentile(lat64, 50) * 1e3), "p99_ms": float(np.percentile(lat64, 99) * 1e3),
-                 "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof64.items()},
-                 "step_roofline_frac": N * args.steps / t64 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9)}
-
-    # dominant phase of the FP32 run by live HIP-event time
-    phases = {k: v for k, v in prof32.items() if k not in ("step", "fd")}
-    dom = max(phases, key=lambda k: phases[k][0])
-    dom_ms = phases[dom][0]
-    if dom == "flow_accum":
-        bytes_per_cell = B_FA["fp32"]
-    elif dom == "mod2":
-        bytes_per_cell = b_mod2(4)
-    else:
-        bytes_per_cell = None
-    roof = {"kernel": dom, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "traffic": None}
-    if bytes_per_cell is not None and dom_ms > 0:
-        ach = bytes_per_cell * N * args.steps / (dom_ms * 1e-3) / 1e9
-        roof.update({"achieved": ach, "frac": ach / HBM_PEAK_GBS,
-                     "bytes_per_cell": bytes_per_cell, "ms_per_step": dom_ms / args.steps,
-                     "launches_per_step": phases[dom][1] / args.steps})
-    else:
-        roof.update({"achieved": None, "frac": None, "ms_per_step": dom_ms / args.steps,
-                     "launches_per_step": phases[dom][1] / args.steps,
-                     "note": "iterative phase without a single-pass byte count"})
+                 "phase_ms_per_step": phase_ms(prof64, args.steps),
+                 "step_roofline_frac": SIDE * SIDE * args.steps / t64 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9),
+                 "roofline": roofline(prof64, args.steps, SIDE * SIDE, 8),
+                 "last_step": {k: s64[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2",
+                                                    "fa_uncomputed", "sed_uncomputed")}}
 
     cpu = None
     if not args.no_cpu:
@@ -155,25 +213,26 @@ def main():
                "sample": f"one FP64 timestep of a {args.cpu_side}x{args.cpu_side} DEM from the same generator "
                          f"(C restatement, OpenMP), {dt:.1f} s", "host": platform.processor() or platform.machine()}
 
-    info = lemhip.device_info(0)
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * t32 / args.steps, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": 1e3 * tb / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 dem/soilT, f32x2 fa)",
         "data": "synthetic",
-        "config": {"workload": f"single {side}x{side} fBm DEM (400 pits, 100 plateaus, lognormal rain), "
-                               "SFD, FP32+Kahan timestep", "side": side, "cells": N,
+        "config": {"workload": f"batch of {args.batch} independent {SIDE}x{SIDE} fBm DEMs (400 pits, 100 plateaus, "
+                               "lognormal rain each) in one atlas grid, SFD, FP32+Kahan timestep",
+                   "dems": args.batch, "side": SIDE, "cells": cells_b, "atlas": list(atlas_dims),
                    "flowdirectiontype": 0, "precision": "fp32_kahan"},
-        "p50_ms": float(np.percentile(lat32, 50) * 1e3), "p99_ms": float(np.percentile(lat32, 99) * 1e3),
+        "p50_ms": float(np.percentile(latb, 50) * 1e3), "p99_ms": float(np.percentile(latb, 99) * 1e3),
         "step_roofline_frac": value * B_ALG["fp32"] / (HBM_PEAK_GBS * 1e9),
-        "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof32.items()},
-        "roofline": roof,
+        "phase_ms_per_step": phase_ms(profb, args.steps),
+        "roofline": roofline(profb, args.steps, atlas_dims[0] * atlas_dims[1], 4),
+        "single_fp32": single32,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
-        "device": info,
-        "last_step": {k: sum32[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2",
-                                             "fa_uncomputed", "sed_uncomputed")},
+        "device": lemhip.device_info(0),
+        "last_step": [{k: s[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2", "fa_uncomputed",
+                                         "sed_uncomputed")} for s in summb[:1]],
     }
     print(json.dumps(out))
 

@@ -145,7 +145,8 @@ int lemhip_step_slots(lemhip_ctx *ctx, const int *iters);
  * routeflow.cu:955-971, mfd_simple.cu:376-380, mfd_accum.cu:474-480,
  * erosion.cu:171-181).  Slots: 0 FD, 1 FlowDirs kernel, 2 plateau rounds,
  * 3 depression routing, 4 runoff kernel, 5 flow accumulation, 6 MOD part 1
- * kernel, 7 sediment routing, 8 MOD part 2 kernel, 9 reductions, 10 step.
+ * kernel, 7 sediment routing, 8 MOD part 2 kernel, 9 reductions, 10 step,
+ * 11 flow-accumulation round kernels only, 12 sediment round kernels only.
  * enable resets the accumulators; read returns milliseconds and launch
  * counts per slot. */
 int lemhip_profile_enable(lemhip_ctx *ctx, int on);

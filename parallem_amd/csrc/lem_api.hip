@@ -204,10 +204,14 @@ static void free_all(lemhip_ctx *c)
 
 static int upload_desc(lemhip_ctx *c)
 {
-    int ns = 0;
+    int ns = 0, kb = 1;
     for (int j = 0; j < LEM_MAX_SLOTS; j++)
-        if (c->hdesc[j].w > 0) ns = j + 1;
+        if (c->hdesc[j].w > 0) {
+            ns = j + 1;
+            if (slot_kb(c->hdesc[j].h) > kb) kb = slot_kb(c->hdesc[j].h);
+        }
     c->nslot = ns > 0 ? ns : 1;
+    c->slot_kbmax = kb;
     LEM_CHECK(c, hipMemcpyAsync(c->ddesc, c->hdesc, sizeof(DemDesc) * (LEM_MAX_SLOTS + 1), hipMemcpyHostToDevice,
                                 c->s));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
@@ -266,7 +270,7 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     LEM_TRY(A((void **)&c->dag_wake, ntiles * 4));
     LEM_TRY(A(&c->dag_ctl, 256));
     LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));
-    LEM_TRY(A((void **)&c->partials, 64L * (LEM_MAX_SLOTS + 1) * 8));
+    LEM_TRY(A((void **)&c->partials, 3L * SLOT_KB * (LEM_MAX_SLOTS + 1) * 8));
     LEM_TRY(A((void **)&c->scal, sizeof(LemScalars)));
     LEM_CHECK(c, hipMemset(c->scal, 0, sizeof(LemScalars)));
     LEM_TRY(A((void **)&c->ddesc, sizeof(DemDesc) * (LEM_MAX_SLOTS + 1)));

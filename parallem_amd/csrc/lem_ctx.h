@@ -10,9 +10,9 @@
 // live timing of selected kernels/phases with HIP events on the context stream
 enum {
     PROF_FD = 0, PROF_FLOWDIRS, PROF_PLATEAU, PROF_FLOOD, PROF_RUNOFF, PROF_FA, PROF_MOD1, PROF_SED,
-    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_COUNT
+    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_FA_K, PROF_SED_K, PROF_COUNT
 };
-#define PROF_MAXP 64
+#define PROF_MAXP 256
 struct ProfSlot {
     hipEvent_t a[PROF_MAXP], b[PROF_MAXP];
     int n = 0;
@@ -47,6 +47,7 @@ struct lemhip_ctx {
     int W = 0, H = 0;     // atlas grid (the DEM itself for a single-DEM context)
     long N = 0;
     int nslot = 1;        // slots (DEMs) of the atlas
+    int slot_kbmax = 1;   // max slot_kb over active slots
     uint8_t *demid = nullptr;     // per-cell slot id (nullptr: one slot covering the grid)
     DemDesc *ddesc = nullptr;     // device copy of hdesc
     DemDesc hdesc[LEM_MAX_SLOTS + 1] = {};   // active rectangles (w == 0: inactive); entry 255 stays empty

@@ -461,7 +461,7 @@ static int grid1d_dag(long n)
 }
 
 template <int TW, int TH, class Cell>
-static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed)
+static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
 {
     int ntiles = ((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
     LEM_CHECK(c, hipMemsetAsync(c->tile_flags, 0, sizeof(int) * ntiles, c->s));
@@ -480,15 +480,17 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed)
     memcpy(&c->hint[16], h, sizeof(h));
     LEM_CHECK(c, hipMemcpyAsync(ctl, &c->hint[16], sizeof(DagCtl), hipMemcpyHostToDevice, c->s));
     const int BATCH = 16;
-    int G = ntiles < 512 ? ntiles : 512;
+    int G = ntiles < 1024 ? ntiles : 1024;
     long guard = 0;
     for (;;) {
         LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));
+        prof_begin(c, prof_slot);
         for (int q = 0; q < BATCH; q++) {
             ++r;
             k_dag_round<TW, TH, Cell><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, c->tile_flags,
                                                            L[r % 3], L[(r + 1) % 3], c->dag_wake, ctl);
         }
+        prof_end(c, prof_slot, BATCH);
         LEM_LAUNCHED(c);
         LEM_CHECK(c, hipMemcpyAsync(&c->hint[20], &ctl->flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
         LEM_CHECK(c, hipStreamSynchronize(c->s));
@@ -515,7 +517,7 @@ int lem_fa_dag(lemhip_ctx *c)
     k_fa_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->dmask, st);
     LEM_LAUNCHED(c);
     FaCell<T, 64, 16> cell{reinterpret_cast<const T *>(c->w), reinterpret_cast<typename FaStore<T>::type *>(c->fa)};
-    return run_rounds<64, 16>(c, cell, &c->h_fa_unc);
+    return run_rounds<64, 16>(c, cell, &c->h_fa_unc, PROF_FA_K);
 }
 
 template <typename T>
@@ -531,7 +533,7 @@ int lem_sed_dag(lemhip_ctx *c)
     SedCell<T, 64, 16> cell{reinterpret_cast<const T *>(c->ero), reinterpret_cast<const T *>(c->inci),
                     reinterpret_cast<const T *>(c->geli), reinterpret_cast<T *>(c->depo),
                     reinterpret_cast<T *>(c->stone), reinterpret_cast<T *>(c->fines), c->soilT, ddk, dck, dgk};
-    return run_rounds<64, 16>(c, cell, &c->h_sed_unc);
+    return run_rounds<64, 16>(c, cell, &c->h_sed_unc, PROF_SED_K);
 }
 
 template int lem_fa_dag<double>(lemhip_ctx *);

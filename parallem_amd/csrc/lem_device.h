@@ -60,6 +60,16 @@ struct DemScal {
     int first_iter;       // this step is the slot's first iteration (mfd_simple.cu:336)
 };
 
+// Per-slot reductions split a slot's rows over slot_kb(h) blocks (row k goes
+// to block k mod slot_kb(h)); the split depends only on the slot's own height,
+// so a DEM's totals do not depend on where it sits in an atlas.
+#define SLOT_KB 1024
+__host__ __device__ inline int slot_kb(int h)
+{
+    int kb = (h + 3) / 4;
+    return kb < 1 ? 1 : (kb > SLOT_KB ? SLOT_KB : kb);
+}
+
 __device__ __forceinline__ int slot_of(const uint8_t *demid, long i) { return demid ? (int)demid[i] : 0; }
 
 // per-step scalars kept on the device

@@ -275,50 +275,90 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
 
 // ------------------------------------------------ per-slot reductions
 //
-// Grid (KB, ns): block (k, j) covers rows y0+k, y0+k+KB, ... of slot s0+j.
-// Sums go through per-block partials combined in a fixed order, so totals are
+// Grid (slot_kbmax, ns): block (k, j) covers rows y0+k, y0+k+kb, ... of slot
+// s0+j with kb = slot_kb(h); blocks k >= kb have nothing to do.  Sums go
+// through per-block partials combined in a fixed order, so totals are
 // reproducible; min/max/first-index are exact under any order.
-
-#define SLOT_KB 64
 
 template <typename F>
 __device__ __forceinline__ void slot_rows(const DemDesc &D, int W, F f)
 {
-    for (int r = D.y0 + blockIdx.x; r < D.y0 + D.h; r += gridDim.x)
+    int kb = slot_kb(D.h);
+    for (int r = D.y0 + blockIdx.x; r < D.y0 + D.h; r += kb)
         for (int c = D.x0 + threadIdx.x; c < D.x0 + D.w; c += blockDim.x) f((long)r * W + c);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_slot_sum(const T *__restrict__ a, int W, const DemDesc *dd, double *partial)
+__device__ __forceinline__ bool slot_block_idle(const DemDesc &D) { return D.w == 0 || (int)blockIdx.x >= slot_kb(D.h); }
+
+__device__ __forceinline__ double block_sum_f64(double v, double *sh)
 {
-    __shared__ double sh[256];
-    DemDesc D = dd[blockIdx.y];
-    double acc = 0.0;
-    if (D.w > 0) slot_rows(D, W, [&](long i) { acc += (double)a[i]; });
-    sh[threadIdx.x] = acc;
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (int t = 128; t > 0; t >>= 1) {
         if (threadIdx.x < t) sh[threadIdx.x] += sh[threadIdx.x + t];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+    double r = sh[0];
+    __syncthreads();
+    return r;
 }
 
-__global__ void k_slot_sum_final(const double *partial, int kb, int ns, DemScal *ds, int off)
+// up to three fields summed in one pass; partial[(j*3 + q)*SLOT_KB + k]
+template <typename T>
+__global__ void __launch_bounds__(256) k_slot_sum3(const T *__restrict__ a0, const T *__restrict__ a1,
+                                                   const T *__restrict__ a2, int W, const DemDesc *dd,
+                                                   double *partial)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    double acc = 0.0;
-    for (int k = 0; k < kb; k++) acc += partial[j * kb + k];
-    *reinterpret_cast<double *>(reinterpret_cast<char *>(&ds[j]) + off) = acc;
+    __shared__ double sh[256];
+    int j = blockIdx.y;
+    DemDesc D = dd[j];
+    if (slot_block_idle(D)) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    slot_rows(D, W, [&](long i) {
+        s0 += (double)a0[i];
+        if (a1) s1 += (double)a1[i];
+        if (a2) s2 += (double)a2[i];
+    });
+    double *P = partial + (long)j * 3 * SLOT_KB + blockIdx.x;
+    s0 = block_sum_f64(s0, sh);
+    if (threadIdx.x == 0) P[0] = s0;
+    if (a1) {
+        s1 = block_sum_f64(s1, sh);
+        if (threadIdx.x == 0) P[SLOT_KB] = s1;
+    }
+    if (a2) {
+        s2 = block_sum_f64(s2, sh);
+        if (threadIdx.x == 0) P[2 * SLOT_KB] = s2;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_slot_sum_final(const double *partial, int na, const DemDesc *dd, DemScal *ds,
+                                                        int off0, int off1, int off2)
+{
+    __shared__ double sh[256];
+    int j = blockIdx.x;
+    DemDesc D = dd[j];
+    if (D.w == 0) return;
+    int kb = slot_kb(D.h);
+    for (int q = 0; q < na; q++) {
+        const double *P = partial + ((long)j * 3 + q) * SLOT_KB;
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < kb; k += blockDim.x) acc += P[k];
+        acc = block_sum_f64(acc, sh);
+        int off = q == 0 ? off0 : (q == 1 ? off1 : off2);
+        if (threadIdx.x == 0) *reinterpret_cast<double *>(reinterpret_cast<char *>(&ds[j]) + off) = acc;
+    }
 }
 
 template <typename T>
-static int slot_sum(lemhip_ctx *c, const void *a, int off)
+static int slot_sum(lemhip_ctx *c, const void *a0, int off0, const void *a1 = nullptr, int off1 = 0,
+                    const void *a2 = nullptr, int off2 = 0)
 {
-    dim3 g(SLOT_KB, c->nslot);
-    k_slot_sum<T><<<g, 256, 0, c->s>>>(reinterpret_cast<const T *>(a), c->W, c->ddesc, c->partials);
-    k_slot_sum_final<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->partials, SLOT_KB, c->nslot, c->dsc, off);
+    dim3 g(c->slot_kbmax, c->nslot);
+    k_slot_sum3<T><<<g, 256, 0, c->s>>>(reinterpret_cast<const T *>(a0), reinterpret_cast<const T *>(a1),
+                                        reinterpret_cast<const T *>(a2), c->W, c->ddesc, c->partials);
+    int na = a2 ? 3 : (a1 ? 2 : 1);
+    k_slot_sum_final<<<c->nslot, 256, 0, c->s>>>(c->partials, na, c->ddesc, c->dsc, off0, off1, off2);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
 }
@@ -352,6 +392,7 @@ __global__ void __launch_bounds__(256) k_slot_lowest(const double *__restrict__ 
     __shared__ unsigned long long sh[256];
     int s = s0 + blockIdx.y;
     DemDesc D = dd[s];
+    if (slot_block_idle(D)) return;
     unsigned long long best = ord_enc(1000.0);
     if (D.w > 0)
         slot_rows(D, W, [&](long i) {
@@ -371,7 +412,7 @@ __global__ void __launch_bounds__(256) k_slot_first_lowest(const double *__restr
 {
     int s = s0 + blockIdx.y;
     DemDesc D = dd[s];
-    if (D.w == 0) return;
+    if (slot_block_idle(D)) return;
     double lowest = ord_dec(ds[s].lowest_enc);
     if (!(lowest < 1000.0)) return;
     slot_rows(D, W, [&](long i) {
@@ -402,7 +443,7 @@ __global__ void k_slots_outlet_from_lowest(int W, const DemDesc *dd, int s0, int
 int lem_slot_lowest(lemhip_ctx *c, const double *z, int s0, int ns)
 {
     k_slots_lowest_init<<<(ns + 255) / 256, 256, 0, c->s>>>(s0, ns, c->dsc);
-    dim3 g(SLOT_KB, ns);
+    dim3 g(c->slot_kbmax, ns);
     k_slot_lowest<<<g, 256, 0, c->s>>>(z, c->mask, c->W, c->ddesc, s0, c->dsc);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
@@ -410,7 +451,7 @@ int lem_slot_lowest(lemhip_ctx *c, const double *z, int s0, int ns)
 
 int lem_slot_first_lowest(lemhip_ctx *c, const double *z, int s0, int ns)
 {
-    dim3 g(SLOT_KB, ns);
+    dim3 g(c->slot_kbmax, ns);
     k_slot_first_lowest<<<g, 256, 0, c->s>>>(z, c->mask, c->W, c->ddesc, s0, c->dsc);
     k_slots_outlet_from_lowest<<<(ns + 255) / 256, 256, 0, c->s>>>(c->W, c->ddesc, s0, ns, c->dsc);
     LEM_LAUNCHED(c);
@@ -427,6 +468,7 @@ __global__ void __launch_bounds__(256) k_slot_famax(const FS *__restrict__ fa, i
     __shared__ unsigned long long sh[256];
     int s = blockIdx.y;
     DemDesc D = dd[s];
+    if (slot_block_idle(D)) return;
     unsigned long long best = ord_enc(0.0);
     if (D.w > 0)
         slot_rows(D, W, [&](long i) {
@@ -443,7 +485,7 @@ __global__ void __launch_bounds__(256) k_slot_fa_first(const FS *__restrict__ fa
 {
     int s = blockIdx.y;
     DemDesc D = dd[s];
-    if (D.w == 0 || !ds[s].first_iter) return;
+    if (slot_block_idle(D) || !ds[s].first_iter) return;
     double m = ord_dec(ds[s].famax_enc);
     if (!(m > 0.0)) return;
     slot_rows(D, W, [&](long i) {
@@ -523,7 +565,7 @@ static int fa_t(lemhip_ctx *c)
     prof_end(c, PROF_FA, c->dag_rounds_last);
     // FA_max and the first-iteration outlet rule (mfd_simple.cu:332-357)
     int ns = c->nslot;
-    dim3 g(SLOT_KB, ns);
+    dim3 g(c->slot_kbmax, ns);
     k_slots_famax_init<<<(ns + 255) / 256, 256, 0, c->s>>>(ns, c->dsc);
     k_slot_famax<FS><<<g, 256, 0, c->s>>>((const FS *)c->fa, c->W, c->ddesc, c->dsc);
     k_slot_fa_first<FS><<<g, 256, 0, c->s>>>((const FS *)c->fa, c->W, c->ddesc, c->dsc);
@@ -553,10 +595,8 @@ static int mod_t(lemhip_ctx *c)
     prof_end(c, PROF_MOD1);
     LEM_LAUNCHED(c);
     prof_begin(c, PROF_REDUCE);
-    LEM_TRY(slot_sum<T>(c, c->ero, DS_OFF(totE)));
-    LEM_TRY(slot_sum<T>(c, c->inci, DS_OFF(totI)));
-    LEM_TRY(slot_sum<T>(c, c->geli, DS_OFF(totG)));
-    prof_end(c, PROF_REDUCE, 3);
+    LEM_TRY(slot_sum<T>(c, c->ero, DS_OFF(totE), c->inci, DS_OFF(totI), c->geli, DS_OFF(totG)));
+    prof_end(c, PROF_REDUCE, 2);
     prof_begin(c, PROF_SED);
     LEM_TRY(lem_sed_dag<T>(c));
     prof_end(c, PROF_SED, c->dag_rounds_last);
@@ -571,9 +611,7 @@ static int mod_t(lemhip_ctx *c)
     prof_end(c, PROF_MOD2);
     k_slots_post<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
     LEM_LAUNCHED(c);
-    LEM_TRY(slot_sum<T>(c, c->wC, DS_OFF(totwC)));
-    LEM_TRY(slot_sum<T>(c, c->wP, DS_OFF(totwP)));
-    LEM_TRY(slot_sum<T>(c, c->TotB, DS_OFF(totbio)));
+    LEM_TRY(slot_sum<T>(c, c->wC, DS_OFF(totwC), c->wP, DS_OFF(totwP), c->TotB, DS_OFF(totbio)));
     return LEMHIP_OK;
 }
 

@@ -45,7 +45,7 @@ EXPORTS = [
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
-              "mod2", "reductions", "step"]
+              "mod2", "reductions", "step", "fa_rounds", "sed_rounds"]
 
 
 class LemError(RuntimeError):
