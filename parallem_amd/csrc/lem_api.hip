@@ -228,6 +228,8 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     c->W = W;
     c->H = H;
     c->N = (long)W * H;
+    c->Hfull = H;
+    c->NP = c->N;
     c->tsz = precision == LEMHIP_FP64 ? 8 : 4;
     if (hipSetDevice(device_id) != hipSuccess) { c->err = "hipSetDevice failed"; return LEMHIP_E_HIP; }
     LEM_CHECK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
@@ -402,7 +404,7 @@ static void launch_init_rect(lemhip_ctx *c, const DemDesc &D, int slot, const do
         (T *)c->temp, (T *)c->last_rain, (T *)c->last_temp, (T *)c->soilM, (T *)c->stone, (T *)c->fines,
         (T *)c->nut, (T *)c->soilB, (T *)c->TotB, (T *)c->w, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci,
         (T *)c->geli, (T *)c->depo, (T *)c->dz, (T *)c->wC, (T *)c->wP, c->fd, c->SFD, (T *)c->slopes, (FS *)c->fa,
-        c->N);
+        c->NP);
 }
 
 // Data.cpp:38-66 (outlet), memory.cu:200-240 and Data.cpp:120-271 (initial
@@ -540,7 +542,7 @@ static size_t kind_elem(const lemhip_ctx *c, int kind)
 // rectangle D of plane `plane` of a grid field <-> contiguous w*h block at `buf`
 static int rect_copy(lemhip_ctx *c, void *buf, void *base, int plane, size_t es, const DemDesc &D, bool down)
 {
-    char *dev = (char *)base + ((size_t)plane * c->N + (size_t)D.y0 * c->W + D.x0) * es;
+    char *dev = (char *)base + ((size_t)plane * c->NP + (size_t)D.y0 * c->W + D.x0) * es;
     char *blk = (char *)buf + (size_t)plane * D.w * D.h * es;
     if (down)
         LEM_CHECK(c, hipMemcpy2DAsync(blk, D.w * es, dev, c->W * es, D.w * es, D.h, hipMemcpyDeviceToHost, c->s));
@@ -781,9 +783,29 @@ static int prof_resolve(lemhip_ctx *c)
     return LEMHIP_OK;
 }
 
+// An atlas steps only the rows its shelves use (rectangles are packed from
+// row 0); the context's H/N describe that band for the duration of the call.
+struct ActiveRows {
+    lemhip_ctx *c;
+    explicit ActiveRows(lemhip_ctx *ctx) : c(ctx)
+    {
+        if (c->atlas) {
+            int h = c->pack.top < 1 ? 1 : (c->pack.top > c->Hfull ? c->Hfull : c->pack.top);
+            c->H = h;
+            c->N = (long)c->W * h;
+        }
+    }
+    ~ActiveRows()
+    {
+        c->H = c->Hfull;
+        c->N = c->NP;
+    }
+};
+
 static int step_impl(lemhip_ctx *c, const int *iters, int iter)
 {
     LEM_CHECK(c, hipSetDevice(c->dev));
+    ActiveRows band(c);
     prof_begin(c, PROF_STEP);
     LEM_TRY(set_first_iter(c, iters, iter));
     LEM_TRY(lem_fd_phase(c, iter));

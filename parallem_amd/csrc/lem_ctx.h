@@ -44,8 +44,10 @@ struct lemhip_ctx {
     int prec = LEMHIP_FP64;
     int dev = 0;
     hipStream_t s = nullptr;
-    int W = 0, H = 0;     // atlas grid (the DEM itself for a single-DEM context)
+    int W = 0, H = 0;     // grid rows/cells a step covers (an atlas steps only its used rows)
     long N = 0;
+    int Hfull = 0;        // allocated rows
+    long NP = 0;          // allocated cells = stride of the Slopes planes
     int nslot = 1;        // slots (DEMs) of the atlas
     int slot_kbmax = 1;   // max slot_kb over active slots
     uint8_t *demid = nullptr;     // per-cell slot id (nullptr: one slot covering the grid)

@@ -647,7 +647,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     dim3 blk(64, 4);
     dim3 grd((W + 63) / 64, (H + 15) / 16);
     prof_begin(c, PROF_FLOWDIRS);
-    k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, N, c->fd, c->SFD,
+    k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, c->NP, c->fd, c->SFD,
                                           c->cand, &c->scal->ncand);
     prof_end(c, PROF_FLOWDIRS);
     LEM_LAUNCHED(c);
@@ -684,7 +684,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
             ++r;
             // round 1 scans every candidate; later frontiers are thin wavefronts
             int g = (r == r0 + 1) ? grid1d(n, 256, 4096) : G;
-            k_plat_scan<TS><<<g, 256, 0, c->s>>>(W, H, r, F[r & 1], zs, sp, c->fd, c->SFD, slopes, N, c->pl_U,
+            k_plat_scan<TS><<<g, 256, 0, c->s>>>(W, H, r, F[r & 1], zs, sp, c->fd, c->SFD, slopes, c->NP, c->pl_U,
                                                  c->pl_Uv, ctl);
             k_plat_next<<<g, 256, 0, c->s>>>(W, H, r, c->pl_U, c->pl_Uv, zs, sp, c->pl_stamp, F[(r + 1) & 1], ctl);
         }

@@ -72,7 +72,7 @@ __global__ void k_runoff(long N, const uint8_t *__restrict__ mask, const T *__re
 // aveslope (eroincidep.cu:8-37), diff_erosion (:39-87), conc_erosion
 // (:89-152), gelifluction (:154-198); depo = 1e-5 seed (mfd_accum.cu:390-394)
 template <typename T>
-__global__ void __launch_bounds__(256) k_mod1(long N, lem_params p, const uint8_t *__restrict__ mask,
+__global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, const uint8_t *__restrict__ mask,
                                               const T *__restrict__ slopes, const typename FaS<T>::type *__restrict__ fa,
                                               const T *__restrict__ fines, const T *__restrict__ nut,
                                               const T *__restrict__ stone, const T *__restrict__ TotB,
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_mod1(long N, lem_params p, const uint8_
         int nonzero = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            T sj = slopes[j * N + self];
+            T sj = slopes[j * plane + self];
             total += sj;
             if (sj > 0) nonzero++;
         }
@@ -588,7 +588,7 @@ static int mod_t(lemhip_ctx *c)
     int ns = c->nslot;
     int g = grid1d(N, 256, 16384);
     prof_begin(c, PROF_MOD1);
-    k_mod1<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa, (const T *)c->fines,
+    k_mod1<T><<<g, 256, 0, c->s>>>(N, c->NP, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa, (const T *)c->fines,
                                    (const T *)c->nut, (const T *)c->stone, (const T *)c->TotB, c->soilT,
                                    (const T *)c->temp, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci, (T *)c->geli,
                                    (T *)c->depo);

@@ -1,0 +1,105 @@
+"""Error bounds of the FP32 + compensated-accumulation form against FP64.
+
+The FP64 reference here is the GPU's plain FP64 form, which is bit-identical
+to the CPU oracle (test_gpu_parity.py); at 4096^2 x 100 steps the CPU oracle
+itself would take hours.  Bounds (stated, and asserted):
+
+* flow accumulation, one step from identical state (identical directions by
+  construction, FD runs on the FP64 dem in both forms): max relative error
+  <= 1e-6 over cells whose FP64 value is below 3e38 (MFD path counts can
+  overflow FP32 otherwise);
+* elevation and flow accumulation over one step from the same state (the
+  FP64 trajectory's state after 50 steps of configuration 2, rounded into the
+  FP32 form): max |dz| <= 1e-4 m, flow accumulation max relative error
+  <= 1e-6 where the directions agree;
+* the 100-step trajectories of configuration 2 (4096^2, SFD) drift apart
+  only through discrete flips: a direction near an exact slope tie, or a
+  branch such as the bare-rock reset at soil thickness 0, taken differently
+  after ~1e-7 relative differences.  Asserted: 99.99 % of cells within
+  1e-4 m after 100 steps, at most 1e-6 of the cells with a different
+  direction at any checked step, every cell within 1e-3 m (the per-step
+  incision cap); the maximum and the mismatch counts are printed.
+"""
+import numpy as np
+import pytest
+
+from parallem_amd import lemhip, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(dem, flowtype):
+    H, W = dem.z.shape
+    out = []
+    for prec in (lemhip.FP64, lemhip.FP32_KAHAN):
+        c = lemhip.LemContext(lemhip.default_params(W, H, flowtype), prec, 0)
+        c.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+        out.append(c)
+    return out
+
+
+def _fa_rel_error(c64, c32, it):
+    for c in (c64, c32):
+        c.flow_direction(it)
+        c.runoff()
+        c.flow_accum(it)
+    fd64, fd32 = c64.download("fd"), c32.download("fd")
+    assert np.array_equal(fd64, fd32), "directions differ on an identical dem"
+    a, b = c64.download("fa"), c32.download("fa")
+    m = (c64.download("mask") == 1) & (np.abs(a) < 3e38) & (a > 0)
+    return float(np.max(np.abs(b[m] - a[m]) / a[m]))
+
+
+@pytest.mark.parametrize("flowtype,side", [(0, 4096), (1, 1024)])
+def test_flow_accumulation_relative_error(flowtype, side):
+    dem = synth.config_dem(2, side=side)
+    c64, c32 = _pair(dem, flowtype)
+    err = _fa_rel_error(c64, c32, -1000)
+    print(f"\n[fp32] flow accumulation max rel error, {side}^2 flowtype {flowtype}: {err:.3e}")
+    assert err <= 1e-6
+
+
+STATE = ["dem", "soilT", "Slopes", "soilM", "stone", "fines", "nut", "soilB", "TotB", "rain", "temp",
+         "last_rain", "last_temp"]
+
+
+def test_one_step_from_the_same_state():
+    dem = synth.config_dem(2)
+    c64, c32 = _pair(dem, 0)
+    for k in range(50):
+        c64.step(-1000 + k)
+    for f in STATE:
+        c32.upload(f, c64.download(f))
+    c32.outlet = c64.outlet
+    c64.step(-950)
+    c32.step(-950)
+    m = c64.download("mask") == 1
+    dz = float(np.max(np.abs(c64.download("dem")[m] - c32.download("dem")[m])))
+    fd_same = c64.download("fd")[m] == c32.download("fd")[m]
+    a, b = c64.download("fa")[m], c32.download("fa")[m]
+    ok = fd_same & (a > 0) & (a < 3e38)
+    rel = float(np.max(np.abs(b[ok] - a[ok]) / a[ok]))
+    print(f"\n[fp32] one step from the FP64 state at step 50: max |dz| = {dz:.3e} m, fa max rel = {rel:.3e}, "
+          f"direction mismatches = {int(np.sum(~fd_same))}")
+    assert dz <= 1e-4
+    assert rel <= 1e-6
+
+
+def test_trajectory_drift_100_steps_config2():
+    dem = synth.config_dem(2)
+    c64, c32 = _pair(dem, 0)
+    m = c64.download("mask") == 1
+    flips = []
+    for k in range(100):
+        c64.step(-1000 + k)
+        c32.step(-1000 + k)
+        if k % 10 == 9 or k == 0:
+            f64, f32 = c64.download("fd"), c32.download("fd")
+            flips.append((k, int(np.sum(f64[m] != f32[m]))))
+    d = np.abs(c64.download("dem")[m] - c32.download("dem")[m])
+    worst, p9999 = float(d.max()), float(np.percentile(d, 99.99))
+    print(f"\n[fp32] config 2, 100 steps: max |dz| = {worst:.3e} m, 99.99th percentile {p9999:.3e} m, "
+          f"cells above 1e-4 m: {int(np.sum(d > 1e-4))}; direction mismatches (step, cells): {flips}")
+    assert p9999 <= 1e-4
+    assert worst <= 1e-3
+    assert max(f for _, f in flips) <= 1e-6 * m.sum()
