@@ -140,6 +140,26 @@ int lemhip_slot_set_climate(lemhip_ctx *ctx, int slot, double rain, double temp)
 int lemhip_slot_summary(lemhip_ctx *ctx, int slot, lem_summary *out);
 int lemhip_step_slots(lemhip_ctx *ctx, const int *iters);
 
+/* Launch tuning (no reference counterpart).  The dependency engine of the
+ * two accumulation passes runs in tiles of dag_tile_w x dag_tile_h cells
+ * (64x16, 32x32, 16x64, 128x8, 64x32 or 32x64) and checks convergence on the
+ * host every dag_batch round launches; plateau routing advances
+ * plateau_rounds_per_launch (4, 8 or 16) synchronous rounds per launch.
+ * Results never depend on these choices.  lemhip_autotune makes the next step
+ * time every candidate on the context's own data (the timed passes are
+ * idempotent), keep the fastest, and record the times in tuned_ms:
+ * [0..5] flow accumulation per tile shape, [8..10] plateau routing per rounds
+ * per launch, [11..13] flow accumulation with batches 4, 8, 32. */
+typedef struct {
+    int dag_tile_w, dag_tile_h;
+    int dag_batch;
+    int plateau_rounds_per_launch;
+    double tuned_ms[16];
+} lem_tuning;
+int lemhip_get_tuning(lemhip_ctx *ctx, lem_tuning *out);
+int lemhip_set_tuning(lemhip_ctx *ctx, const lem_tuning *t);
+int lemhip_autotune(lemhip_ctx *ctx);
+
 /* Live timing with HIP events on the context stream (no reference
  * counterpart; the reference logs phase times from cudaEvents at
  * routeflow.cu:955-971, mfd_simple.cu:376-380, mfd_accum.cu:474-480,

@@ -181,7 +181,7 @@ static void free_all(lemhip_ctx *c)
     void *ptrs[] = {c->dem, c->soilT, c->mask, c->slopes, c->soilM, c->stone, c->fines, c->nut, c->soilB, c->TotB,
                     c->rain, c->temp, c->last_rain, c->last_temp, c->fd, c->SFD, c->w, c->fa, c->slopeptr, c->ero,
                     c->inci, c->geli, c->depo, c->dz, c->wC, c->wP, c->sp0, c->sp1, c->cand, c->ws, c->scan_tmp,
-                    c->z2, c->pl_front0, c->pl_front1, c->pl_U, c->pl_stamp, c->pl_Uv, c->pl_ctl, c->cub_tmp,
+                    c->z2, c->pl_tiles, c->pl_ctl, c->cub_tmp,
                     c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
                     c->dmask, c->tile_flags, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
                     c->dag_ctl, c->partials, c->scal, c->demid, c->ddesc, c->dsc, c->slot_flags, c->labdem,
@@ -253,25 +253,29 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     LEM_TRY(A(&c->fa, N * 8));  // double, or float2 hi/lo
     LEM_TRY(A((void **)&c->sp0, N * 4));
     LEM_TRY(A((void **)&c->sp1, N * 4));
-    LEM_TRY(A((void **)&c->pl_front0, N * 4));
-    LEM_TRY(A((void **)&c->pl_front1, N * 4));
-    LEM_TRY(A((void **)&c->pl_U, N * 4));
-    LEM_TRY(A((void **)&c->pl_Uv, N * 4));
-    LEM_TRY(A((void **)&c->pl_stamp, N * 4));
+    long ptiles = (long)((W + PTW - 1) / PTW) * ((H + PTH - 1) / PTH);
+    LEM_TRY(A((void **)&c->pl_tiles, 5 * ptiles * 4));
     LEM_TRY(A(&c->pl_ctl, 256));
-    LEM_CHECK(c, hipMemset(c->pl_stamp, 0, N * 4));
+    LEM_CHECK(c, hipMemset(c->pl_tiles, 0, 5 * ptiles * 4));
     LEM_TRY(A((void **)&c->cand, N * 4));
     LEM_TRY(A((void **)&c->ws, N * 4));
     LEM_TRY(A((void **)&c->scan_tmp, N * 4));
     LEM_TRY(A((void **)&c->z2, N * 8));
     LEM_TRY(A((void **)&c->dmask, N));
     LEM_TRY(A((void **)&c->labdem, N + 1));
-    long ntiles = ((c->W + 63) / 64) * (long)((c->H + 15) / 16);
+    long ntiles = 0;
+    for (int k = 0; k < lem_dag_shape_count(); k++) {
+        int tw, th;
+        lem_dag_shape_dims(k, &tw, &th);
+        long nt = ((c->W + tw - 1) / tw) * (long)((c->H + th - 1) / th);
+        if (nt > ntiles) ntiles = nt;
+    }
+    c->dag_tiles_max = ntiles;
     LEM_TRY(A((void **)&c->tile_flags, ntiles * 4));
     for (int q = 0; q < 3; q++) LEM_TRY(A((void **)&c->dag_list[q], ntiles * 4));
     LEM_TRY(A((void **)&c->dag_wake, ntiles * 4));
-    LEM_TRY(A(&c->dag_ctl, 256));
     LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));
+    LEM_TRY(A(&c->dag_ctl, 256));
     LEM_TRY(A((void **)&c->partials, 3L * SLOT_KB * (LEM_MAX_SLOTS + 1) * 8));
     LEM_TRY(A((void **)&c->scal, sizeof(LemScalars)));
     LEM_CHECK(c, hipMemset(c->scal, 0, sizeof(LemScalars)));
@@ -814,7 +818,44 @@ static int step_impl(lemhip_ctx *c, const int *iters, int iter)
     LEM_TRY(lem_mod_phase(c, iter));
     prof_end(c, PROF_STEP);
     LEM_CHECK(c, hipStreamSynchronize(c->s));
+    c->tune_pending = false;
     LEM_TRY(prof_resolve(c));
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_get_tuning(lemhip_ctx *c, lem_tuning *o)
+{
+    if (!c || !o) return LEMHIP_E_ARG;
+    lem_dag_shape_dims(c->dag_shape, &o->dag_tile_w, &o->dag_tile_h);
+    o->dag_batch = c->dag_batch;
+    o->plateau_rounds_per_launch = c->plat_k;
+    for (int k = 0; k < 16; k++) o->tuned_ms[k] = c->tune_ms[k];
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
+{
+    if (!c || !t) return LEMHIP_E_ARG;
+    int shape = -1;
+    for (int k = 0; k < lem_dag_shape_count(); k++) {
+        int tw, th;
+        lem_dag_shape_dims(k, &tw, &th);
+        if (tw == t->dag_tile_w && th == t->dag_tile_h) shape = k;
+    }
+    if (shape < 0) { c->err = "unsupported tile shape"; return LEMHIP_E_ARG; }
+    if (t->dag_batch < 1 || t->dag_batch > 256) { c->err = "dag_batch out of range"; return LEMHIP_E_ARG; }
+    int k = t->plateau_rounds_per_launch;
+    if (k != 4 && k != 8 && k != 16) { c->err = "plateau rounds per launch must be 4, 8 or 16"; return LEMHIP_E_ARG; }
+    c->dag_shape = shape;
+    c->dag_batch = t->dag_batch;
+    c->plat_k = k;
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_autotune(lemhip_ctx *c)
+{
+    if (!c) return LEMHIP_E_ARG;
+    c->tune_pending = true;
     return LEMHIP_OK;
 }
 

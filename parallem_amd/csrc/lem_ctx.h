@@ -21,6 +21,10 @@ struct ProfSlot {
     long events = 0;
 };
 
+// plateau tiles: PTW x PTH cells advanced plat_k synchronous rounds per launch
+#define PTW 64
+#define PTH 32
+
 // per-cell status of the dependency engine
 enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2 };
 
@@ -83,11 +87,10 @@ struct lemhip_ctx {
     void *dz = nullptr, *wC = nullptr, *wP = nullptr;
 
     // ---- scratch ----
-    float *sp0 = nullptr, *sp1 = nullptr;
-    int *pl_front0 = nullptr, *pl_front1 = nullptr, *pl_U = nullptr, *pl_stamp = nullptr;
-    float *pl_Uv = nullptr;
+    float *sp0 = nullptr, *sp1 = nullptr;   // plateau distances: published / written by the current launch
+    int *pl_tiles = nullptr;                 // per plateau tile: has-candidates, stamp, 3 rotating lists
     void *pl_ctl = nullptr;
-    int pl_round = 0;
+    int pl_launch = 0;
     int *cand = nullptr;       // plateau candidates (also holds pass-2 list)
     int *ws = nullptr;         // watershed labels
     int *scan_tmp = nullptr;   // sink flags / prefix
@@ -107,11 +110,17 @@ struct lemhip_ctx {
     uint8_t *status = nullptr;  // DAG engine status
     uint8_t *dmask = nullptr;   // DAG donor bits (pull order)
     int *tile_flags = nullptr;  // per tile: 1 = finished
-    int *dag_list[3] = {nullptr, nullptr, nullptr};
-    int *dag_wake = nullptr;
+    int *dag_list[3] = {nullptr, nullptr, nullptr};   // rotating lists of woken tiles
+    int *dag_wake = nullptr;                          // per tile: last round it was queued for
     void *dag_ctl = nullptr;
     int dag_round = 0;
     int dag_rounds_last = 0;
+    int plat_k = 8;             // plateau rounds per launch (4, 8 or 16)
+    bool tune_pending = false;  // the next step times the candidates and keeps the fastest
+    double tune_ms[16] = {};    // last tuning: per-candidate times
+    int dag_shape = 0;          // tile shape id of the dependency engine
+    int dag_batch = 16;         // round launches between host checks
+    long dag_tiles_max = 0;     // tile arrays are sized for the shape with the most tiles
     double *partials = nullptr; // deterministic reductions
     LemScalars *scal = nullptr;   // device
     LemScalars *hscal = nullptr;  // pinned host mirror
@@ -184,6 +193,9 @@ int lem_sync_scalars(lemhip_ctx *c);
 int lem_slot_lowest(lemhip_ctx *c, const double *z, int s0, int ns);
 int lem_slot_first_lowest(lemhip_ctx *c, const double *z, int s0, int ns);
 int lem_read_slots(lemhip_ctx *c);
+int lem_dag_shape_count();
+int lem_tune_dag(lemhip_ctx *c);
+void lem_dag_shape_dims(int id, int *tw, int *th);
 static inline int lem_grid1d(long n, int block = 256, int cap = 8192)
 {
     long g = (n + block - 1) / block;

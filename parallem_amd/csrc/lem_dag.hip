@@ -35,76 +35,90 @@ __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
 
 // ------------------------------------------------------------ cell bodies
 //
-// A cell body caches, per tile cell, what its receivers in the same tile will
-// read (LDS), and evaluates one cell once all donors are final.  Donor values
-// come from LDS for donors inside the tile and from HBM for donors in other
-// tiles (completed in an earlier launch).
+// A cell body caches in LDS, per tile cell, what its receivers in the same
+// tile will read, plus the same values for the one-cell ring around the tile
+// (donors in other tiles, completed in an earlier launch), and evaluates one
+// cell once all donors are final.  Walks then never wait on global loads.
+
+// ring index of a neighbour (nr, nc) just outside a TW x TH tile: top row,
+// bottom row (both with corners), left column, right column
+template <int TW, int TH>
+__device__ __forceinline__ int halo_idx(int nr, int nc)
+{
+    if (nr < 0) return nc + 1;
+    if (nr >= TH) return (TW + 2) + nc + 1;
+    if (nc < 0) return 2 * (TW + 2) + nr;
+    return 2 * (TW + 2) + TH + nr;
+}
+template <int TW, int TH>
+__device__ __forceinline__ void halo_pos(int h, int *nr, int *nc)
+{
+    if (h < TW + 2) { *nr = -1; *nc = h - 1; }
+    else if (h < 2 * (TW + 2)) { *nr = TH; *nc = h - (TW + 2) - 1; }
+    else if (h < 2 * (TW + 2) + TH) { *nr = h - 2 * (TW + 2); *nc = -1; }
+    else { *nr = h - 2 * (TW + 2) - TH; *nc = TW; }
+}
+
+__device__ __forceinline__ void fa_acc(double &a, double v) { a += v * 1.0; }
+__device__ __forceinline__ void fa_acc(float2 &a, float2 v) { a = ff_add(a, v); }
+__device__ __forceinline__ void fa_clamp(double &a) { if (a < 0.0) a = 0.0; }
+__device__ __forceinline__ void fa_clamp(float2 &a) { if (a.x < 0.0f) a = make_float2(0.f, 0.f); }
+__device__ __forceinline__ double fa_final(double w, double a) { double v = w; v += a; return v; }
+__device__ __forceinline__ float2 fa_final(float w, float2 a) { return ff_add(make_float2(w, 0.f), a); }
 
 template <typename T, int TW, int TH>
 struct FaCell {
     using FS = typename FaStore<T>::type;
     static constexpr int NC = TW * TH;
+    static constexpr int NH = 2 * (TW + 2) + 2 * TH;
     static constexpr bool kCanDie = false;
     const T *w;
     FS *fa;
     struct Lds {
         FS val[NC];
         T w[NC];
+        FS hval[NH];
     };
-    __device__ __forceinline__ void preload(Lds &L, int li, long i, bool done) const
+    // loads are issued unconditionally and stored to LDS after one wait; the
+    // value of a cell that is not final yet is never read
+    struct Reg {
+        T w;
+        FS v;
+    };
+    __device__ __forceinline__ Reg load(long i) const { return Reg{w[i], fa[i]}; }
+    __device__ __forceinline__ void store(Lds &L, int li, const Reg &r) const
     {
-        L.w[li] = w[i];
-        if (done) L.val[li] = fa[i];
+        L.w[li] = r.w;
+        L.val[li] = r.v;
     }
+    __device__ __forceinline__ Reg load_halo(long i) const { return Reg{T(0), fa[i]}; }
+    __device__ __forceinline__ void store_halo(Lds &L, int h, const Reg &r) const { L.hval[h] = r.v; }
     // fa = w + max(0, sum of donor fa in the order E..NE) (mfd_simple.cu:71-168;
     // prop = 1 for every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
-    __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const;
+    __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const
+    {
+        (void)W;
+        FS accum{};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (bits & (1u << k)) {
+                int d = kPULL[k];
+                int nr = lr + kDY[d], nc = lc + kDX[d];
+                FS v = (nr >= 0 && nr < TH && nc >= 0 && nc < TW) ? L.val[nr * TW + nc] : L.hval[halo_idx<TW, TH>(nr, nc)];
+                fa_acc(accum, v);
+            }
+        }
+        fa_clamp(accum);
+        FS v = fa_final(L.w[li], accum);
+        fa[i] = v;
+        L.val[li] = v;
+    }
 };
-
-template <>
-__device__ __forceinline__ void FaCell<double, 64, 16>::compute(Lds &L, int li, long i, int W, unsigned bits, int lr,
-                                                                int lc) const
-{
-    double accum = 0.;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (bits & (1u << k)) {
-            int d = kPULL[k];
-            int nr = lr + kDY[d], nc = lc + kDX[d];
-            double v = (nr >= 0 && nr < 16 && nc >= 0 && nc < 64) ? L.val[nr * 64 + nc] : fa[i + (long)kDY[d] * W + kDX[d]];
-            accum += v * 1.0;
-        }
-    }
-    if (accum < 0.0) accum = 0.0;
-    double v = L.w[li];
-    v += accum;
-    fa[i] = v;
-    L.val[li] = v;
-}
-
-template <>
-__device__ __forceinline__ void FaCell<float, 64, 16>::compute(Lds &L, int li, long i, int W, unsigned bits, int lr,
-                                                               int lc) const
-{
-    float2 accum = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (bits & (1u << k)) {
-            int d = kPULL[k];
-            int nr = lr + kDY[d], nc = lc + kDX[d];
-            float2 v = (nr >= 0 && nr < 16 && nc >= 0 && nc < 64) ? L.val[nr * 64 + nc] : fa[i + (long)kDY[d] * W + kDX[d]];
-            accum = ff_add(accum, v);
-        }
-    }
-    if (accum.x < 0.0f) accum = make_float2(0.f, 0.f);
-    float2 v = ff_add(make_float2(L.w[li], 0.f), accum);
-    fa[i] = v;
-    L.val[li] = v;
-}
 
 template <typename T, int TW, int TH>
 struct SedCell {
     static constexpr int NC = TW * TH;
+    static constexpr int NH = 2 * (TW + 2) + 2 * TH;
     static constexpr bool kCanDie = true;
     const T *ero, *inci, *geli;
     T *depo, *stone, *fines;
@@ -112,18 +126,31 @@ struct SedCell {
     double ddk, dck, dgk;
     struct Lds {
         T ero[NC], inci[NC], geli[NC], stone[NC];
+        T hero[NH], hinci[NH], hgeli[NH], hstone[NH];
     };
-    __device__ __forceinline__ void preload(Lds &L, int li, long i, bool done) const
+    struct Reg {
+        T e, in, g, s;
+    };
+    __device__ __forceinline__ Reg load(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i]}; }
+    __device__ __forceinline__ void store(Lds &L, int li, const Reg &r) const
     {
-        (void)done;
-        L.ero[li] = ero[i];
-        L.inci[li] = inci[i];
-        L.geli[li] = geli[i];
-        L.stone[li] = stone[i];
+        L.ero[li] = r.e;
+        L.inci[li] = r.in;
+        L.geli[li] = r.g;
+        L.stone[li] = r.s;
+    }
+    __device__ __forceinline__ Reg load_halo(long i) const { return load(i); }
+    __device__ __forceinline__ void store_halo(Lds &L, int h, const Reg &r) const
+    {
+        L.hero[h] = r.e;
+        L.hinci[h] = r.in;
+        L.hgeli[h] = r.g;
+        L.hstone[h] = r.s;
     }
     // MFDsedbud body (mfd_accum.cu:63-252); donors' stone is their updated value
     __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const
     {
+        (void)W;
         T accum_d = 0, accum_c = 0, accum_g = 0;
         T from[8];
         const T one = (T)1;
@@ -139,8 +166,8 @@ struct SedCell {
                     int nl = nr * TW + nc;
                     e = L.ero[nl]; in = L.inci[nl]; g = L.geli[nl]; sn = L.stone[nl];
                 } else {
-                    long n = i + (long)kDY[d] * W + kDX[d];
-                    e = ero[n]; in = inci[n]; g = geli[n]; sn = stone[n];
+                    int h = halo_idx<TW, TH>(nr, nc);
+                    e = L.hero[h]; in = L.hinci[h]; g = L.hgeli[h]; sn = L.hstone[h];
                 }
                 accum_d += one * (e * kd);
                 accum_c += one * (in * kc);
@@ -238,12 +265,15 @@ __global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const
 
 // ------------------------------------------------------------ tiled rounds
 
-// Round bookkeeping: the active tile list of round r is L[r % 3] with count
-// cnt[r % 3]; round r appends the tiles it wakes to L[(r+1) % 3] and clears
-// cnt[(r+2) % 3] (last read by round r-1, next filled by round r+1).
+// Round bookkeeping: the woken tiles of round r are list L[r % 3] with count
+// cnt[r % 3]; round r appends the tiles it wakes to L[(r+1) % 3] (each once:
+// wake[tile] holds the last round it was queued for) and clears cnt[(r+2) % 3],
+// last read by round r-1.  flag records progress for the host.
 struct DagCtl {
     int cnt[3];
+    int pad0[29];
     int flag;
+    int pad1[31];
 };
 
 #define DAG_BIG (1 << 24)
@@ -254,14 +284,16 @@ struct DagCtl {
 // another thread needs only the LDS queue drained, not the global stores.
 __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-// One round over the active tiles.  Inside a tile every cell gets a counter of
+// One round over the woken tiles.  Inside a tile every cell gets a counter of
 // donors still unresolved in the tile; cells whose donors in other tiles are
 // not all final from earlier launches are parked for a later round.  A thread
 // resolving a cell decrements its receivers in the tile, and the thread that
 // brings a receiver's counter to zero continues along it, so a dependency
 // chain runs inside the tile at LDS speed with no barrier per link.  Extra
 // ready receivers (MFD) go to an LDS spill list processed after a barrier.
-// A cell with a dead donor dies (sediment blocking, D9).
+// A chain leaving the tile only marks the neighbour tile in an LDS mask; the
+// tile wakes each marked neighbour once, after its walks.  A cell with a dead
+// donor dies (sediment blocking, D9).
 template <int TW, int TH, class Cell>
 __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
                                                    const uint8_t *__restrict__ fdv, int *__restrict__ st,
@@ -270,42 +302,91 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
 {
     constexpr int NC = TW * TH;
     constexpr int PER = NC / 256;
+    constexpr int NH = Cell::NH;
+    constexpr int HPER = (NH + 255) / 256;
+    static_assert(NC % 256 == 0, "tile shape");
     __shared__ int lcnt[NC];
     __shared__ uint8_t lst[NC];
     __shared__ uint8_t ldm[NC];
     __shared__ uint8_t lfd[NC];
+    __shared__ uint8_t hst[NH];
     __shared__ typename Cell::Lds lds;
     __shared__ int spill[2][NC];
     __shared__ int nspill[2];
     __shared__ int any_pending;
+    __shared__ unsigned exits;
     const int n = ctl->cnt[round % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(round + 2) % 3] = 0;
-    const int ntx = (W + TW - 1) / TW;
-    for (int t = blockIdx.x; t < n; t += gridDim.x) {
-        const int tile = act[t];
+    const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH;
+    for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
+        const int tile = act[q0];
         if (tile_done[tile]) continue;
         const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
         __syncthreads();
-        if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; }
-        // 1. load states and the tile's cached values
+        if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; exits = 0; }
+        // 1. load states and cached values of the tile and of its ring (all
+        //    loads in flight before the first wait); a ring cell counts as
+        //    final only if it completed in an earlier launch
+        int vs[PER];
+        uint8_t vb[PER], vf[PER];
+        typename Cell::Reg vr[PER];
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             int li = threadIdx.x + q * 256;
             int r = ty0 + li / TW, c = tx0 + li % TW;
-            uint8_t sv = ST_DONE;
-            uint8_t b = 0, f = 0;
+            vs[q] = 0;   // off-grid: done before the first round, never a donor or receiver
+            vb[q] = 0;
+            vf[q] = 0;
+            vr[q] = typename Cell::Reg{};
             if (r < H && c < W) {
                 long i = (long)r * W + c;
-                int v = st[i];
-                sv = (v == DAG_PENDING) ? ST_PENDING : (v >= 0 ? ST_DONE : ST_DEAD);
-                cell.preload(lds, li, i, sv == ST_DONE);
-                b = dm[i];
-                f = fdv[i];
+                vs[q] = st[i];
+                vb[q] = dm[i];
+                vf[q] = fdv[i];
+                vr[q] = cell.load(i);
             }
+        }
+        int hv[HPER];
+        typename Cell::Reg hr[HPER];
+#pragma unroll
+        for (int q = 0; q < HPER; q++) {
+            int hh = threadIdx.x + q * 256;
+            hv[q] = 0;   // off-grid ring cells are never donors
+            hr[q] = typename Cell::Reg{};
+            if (hh < NH) {
+                int nr, nc;
+                halo_pos<TW, TH>(hh, &nr, &nc);
+                int r = ty0 + nr, c = tx0 + nc;
+                if (r >= 0 && r < H && c >= 0 && c < W) {
+                    long i = (long)r * W + c;
+                    hv[q] = st[i];
+                    hr[q] = cell.load_halo(i);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int li = threadIdx.x + q * 256;
+            int v = vs[q];
+            uint8_t sv = (v == DAG_PENDING) ? ST_PENDING : (v >= 0 ? ST_DONE : ST_DEAD);
+            cell.store(lds, li, vr[q]);
             lst[li] = sv;
-            ldm[li] = b;
-            lfd[li] = f;
+            ldm[li] = vb[q];
+            lfd[li] = vf[q];
             lcnt[li] = DAG_BIG;
+        }
+#pragma unroll
+        for (int q = 0; q < HPER; q++) {
+            int hh = threadIdx.x + q * 256;
+            if (hh < NH) {
+                int v = hv[q];
+                uint8_t hs;
+                if (v == DAG_PENDING) hs = ST_PENDING;
+                else if (v >= 0) hs = (v < round) ? ST_DONE : ST_PENDING;
+                else hs = ((-v - 1) < round) ? ST_DEAD : ST_PENDING;
+                hst[hh] = hs;
+                cell.store_halo(lds, hh, hr[q]);
+            }
         }
         __syncthreads();
         // 2. counters: donors unresolved in the tile; park cells waiting on other tiles
@@ -316,7 +397,6 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
             int li = threadIdx.x + q * 256;
             if (lst[li] != ST_PENDING) continue;
             int lr = li / TW, lc = li % TW;
-            long i = (long)(ty0 + lr) * W + (tx0 + lc);
             unsigned bits = ldm[li];
             int nin = 0;
             bool dead = false, parked = false;
@@ -325,23 +405,17 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
                 if (!(bits & (1u << k))) continue;
                 int d = kPULL[k];
                 int nr = lr + kDY[d], nc = lc + kDX[d];
-                if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
-                    uint8_t s2 = lst[nr * TW + nc];
-                    if (s2 == ST_PENDING) nin++;
-                    else if (s2 == ST_DEAD) dead = true;
-                } else {
-                    int v = st[i + (long)kDY[d] * W + kDX[d]];
-                    if (v == DAG_PENDING) parked = true;
-                    else if (v >= 0) { if (v >= round) parked = true; }
-                    else { if ((-v - 1) < round) dead = true; else parked = true; }
-                }
+                bool inside = nr >= 0 && nr < TH && nc >= 0 && nc < TW;
+                uint8_t s2 = inside ? lst[nr * TW + nc] : hst[halo_idx<TW, TH>(nr, nc)];
+                if (s2 == ST_PENDING) { if (inside) nin++; else parked = true; }
+                else if (s2 == ST_DEAD) dead = true;
             }
             if (dead) { lcnt[li] = DAG_BIG; seeds |= 1 << q; deadseeds |= 1 << q; }
             else if (!parked) { lcnt[li] = nin; if (nin == 0) seeds |= 1 << q; }
         }
         __syncthreads();
         // 3. walks
-        bool wake_any = false;
+        unsigned my_exits = 0;
         int pass = 0;
         for (;;) {
             int nwork = (pass == 0) ? 0 : nspill[(pass - 1) & 1];
@@ -400,13 +474,9 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
                                 else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = rl;
                             }
                         } else {
-                            int gy = ty0 + nr, gx = tx0 + nc;
-                            if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
-                                int nt = (gy / TH) * ntx + (gx / TW);
-                                if (!tile_done[nt] && atomicExch(&wake[nt], round + 1) != round + 1)
-                                    next[atomicAdd(&ctl->cnt[(round + 1) % 3], 1)] = nt;
-                                wake_any = true;
-                            }
+                            // receiver in a neighbour tile: 3x3 position of that tile
+                            int ty = nr < 0 ? 0 : (nr >= TH ? 2 : 1), tx = nc < 0 ? 0 : (nc >= TW ? 2 : 1);
+                            my_exits |= 1u << (ty * 3 + tx);
                         }
                     }
                     cur = nxt;
@@ -417,24 +487,35 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
             __syncthreads();
             pass++;
         }
+        if (my_exits) atomicOr(&exits, my_exits);
         // 4. bookkeeping
         bool pend = false;
 #pragma unroll
         for (int q = 0; q < PER; q++) pend |= (lst[threadIdx.x + q * 256] == ST_PENDING);
         if (pend) any_pending = 1;
         int prog = __syncthreads_or(seeds != 0);
-        (void)wake_any;
+        // wake each neighbour the chains flow into, once per round; wave 0
+        // appends its picks with one atomic
+        if (threadIdx.x < 64) {
+            int t2 = -1;
+            if (threadIdx.x < 9 && ((exits >> threadIdx.x) & 1)) {
+                int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+                if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
+                    t2 = ty * ntx + tx;
+                    if (atomicExch(&wake[t2], round + 1) == round + 1) t2 = -1;
+                }
+            }
+            unsigned long long m = __ballot(t2 >= 0);
+            int base = 0;
+            if (threadIdx.x == 0 && m) base = atomicAdd(&ctl->cnt[(round + 1) % 3], __popcll(m));
+            base = __shfl(base, 0);
+            if (t2 >= 0) next[base + __popcll(m & ((1ULL << threadIdx.x) - 1))] = t2;
+        }
         if (threadIdx.x == 0) {
             if (prog) ctl->flag = 1;
             if (!any_pending) tile_done[tile] = 1;
         }
     }
-}
-
-__global__ void k_iota(int n, int *a)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = i;
 }
 
 __global__ void k_count_pending(long N, const uint8_t *mask, const int *st, int *out)
@@ -460,6 +541,12 @@ static int grid1d_dag(long n)
     return (int)g;
 }
 
+__global__ void k_iota(int n, int *a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
 template <int TW, int TH, class Cell>
 static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
 {
@@ -470,17 +557,17 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
     DagCtl *ctl = reinterpret_cast<DagCtl *>(c->dag_ctl);
     int r = c->dag_round;
     if (r > (1 << 29)) {  // wake stamps and status words hold round numbers
-        LEM_CHECK(c, hipMemsetAsync(c->dag_wake, 0, sizeof(int) * ntiles, c->s));
+        LEM_CHECK(c, hipMemsetAsync(c->dag_wake, 0, sizeof(int) * c->dag_tiles_max, c->s));
         r = 0;
     }
+    // the first round visits every tile
     k_iota<<<(ntiles + 255) / 256, 256, 0, c->s>>>(ntiles, L[(r + 1) % 3]);
     LEM_LAUNCHED(c);
-    int h[4] = {0, 0, 0, 0};
-    h[(r + 1) % 3] = ntiles;
-    memcpy(&c->hint[16], h, sizeof(h));
-    LEM_CHECK(c, hipMemcpyAsync(ctl, &c->hint[16], sizeof(DagCtl), hipMemcpyHostToDevice, c->s));
-    const int BATCH = 16;
-    int G = ntiles < 1024 ? ntiles : 1024;
+    LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(DagCtl), c->s));
+    c->hint[16] = ntiles;
+    LEM_CHECK(c, hipMemcpyAsync(&ctl->cnt[(r + 1) % 3], &c->hint[16], sizeof(int), hipMemcpyHostToDevice, c->s));
+    const int BATCH = c->dag_batch;
+    const int G = ntiles < 1024 ? ntiles : 1024;
     long guard = 0;
     for (;;) {
         LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));
@@ -509,6 +596,30 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
     return LEMHIP_OK;
 }
 
+// Tile shapes of the dependency engine (the autotuner picks one per context).
+// Every shape gives identical results: the per-cell arithmetic and its order
+// do not depend on the tiling, only the number of rounds does.
+#define DAG_SHAPES(X) X(0, 64, 16) X(1, 32, 32) X(2, 16, 64) X(3, 128, 8) X(4, 64, 32) X(5, 32, 64)
+
+template <typename T, int TW, int TH>
+static int fa_shape(lemhip_ctx *c)
+{
+    FaCell<T, TW, TH> cell{reinterpret_cast<const T *>(c->w), reinterpret_cast<typename FaStore<T>::type *>(c->fa)};
+    return run_rounds<TW, TH>(c, cell, &c->h_fa_unc, PROF_FA_K);
+}
+
+template <typename T, int TW, int TH>
+static int sed_shape(lemhip_ctx *c)
+{
+    double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
+    if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407
+    if (dck > 0.9) dck = 0.9;
+    SedCell<T, TW, TH> cell{reinterpret_cast<const T *>(c->ero), reinterpret_cast<const T *>(c->inci),
+                            reinterpret_cast<const T *>(c->geli), reinterpret_cast<T *>(c->depo),
+                            reinterpret_cast<T *>(c->stone), reinterpret_cast<T *>(c->fines), c->soilT, ddk, dck, dgk};
+    return run_rounds<TW, TH>(c, cell, &c->h_sed_unc, PROF_SED_K);
+}
+
 template <typename T>
 int lem_fa_dag(lemhip_ctx *c)
 {
@@ -516,8 +627,12 @@ int lem_fa_dag(lemhip_ctx *c)
     int *st = reinterpret_cast<int *>(c->scan_tmp);
     k_fa_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->dmask, st);
     LEM_LAUNCHED(c);
-    FaCell<T, 64, 16> cell{reinterpret_cast<const T *>(c->w), reinterpret_cast<typename FaStore<T>::type *>(c->fa)};
-    return run_rounds<64, 16>(c, cell, &c->h_fa_unc, PROF_FA_K);
+    switch (c->dag_shape) {
+#define X(id, tw, th) case id: return fa_shape<T, tw, th>(c);
+    DAG_SHAPES(X)
+#undef X
+    default: c->err = "unknown tile shape"; return LEMHIP_E_ARG;
+    }
 }
 
 template <typename T>
@@ -527,13 +642,66 @@ int lem_sed_dag(lemhip_ctx *c)
     int *st = reinterpret_cast<int *>(c->scan_tmp);
     k_sed_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->demid, c->ddesc, c->dmask, st);
     LEM_LAUNCHED(c);
-    double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
-    if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407
-    if (dck > 0.9) dck = 0.9;
-    SedCell<T, 64, 16> cell{reinterpret_cast<const T *>(c->ero), reinterpret_cast<const T *>(c->inci),
-                    reinterpret_cast<const T *>(c->geli), reinterpret_cast<T *>(c->depo),
-                    reinterpret_cast<T *>(c->stone), reinterpret_cast<T *>(c->fines), c->soilT, ddk, dck, dgk};
-    return run_rounds<64, 16>(c, cell, &c->h_sed_unc, PROF_SED_K);
+    switch (c->dag_shape) {
+#define X(id, tw, th) case id: return sed_shape<T, tw, th>(c);
+    DAG_SHAPES(X)
+#undef X
+    default: c->err = "unknown tile shape"; return LEMHIP_E_ARG;
+    }
+}
+
+// Times flow accumulation with every tile shape, then the best shape with
+// several launch batches, on the current directions and runoff weights (the
+// pass is idempotent), and keeps the fastest.  Results do not depend on the
+// choice.
+int lem_tune_dag(lemhip_ctx *c)
+{
+    hipEvent_t e0, e1;
+    LEM_CHECK(c, hipEventCreate(&e0));
+    LEM_CHECK(c, hipEventCreate(&e1));
+    auto time_one = [&](float *ms) -> int {
+        LEM_CHECK(c, hipEventRecord(e0, c->s));
+        LEM_TRY(c->prec == LEMHIP_FP64 ? lem_fa_dag<double>(c) : lem_fa_dag<float>(c));
+        LEM_CHECK(c, hipEventRecord(e1, c->s));
+        LEM_CHECK(c, hipEventSynchronize(e1));
+        LEM_CHECK(c, hipEventElapsedTime(ms, e0, e1));
+        return LEMHIP_OK;
+    };
+    float best = 1e30f;
+    int bs = c->dag_shape;
+    c->dag_batch = 16;
+    for (int k = 0; k < lem_dag_shape_count(); k++) {
+        c->dag_shape = k;
+        float ms = 0.f;
+        LEM_TRY(time_one(&ms));
+        c->tune_ms[k] = ms;
+        if (ms < best) { best = ms; bs = k; }
+    }
+    c->dag_shape = bs;
+    static const int BATCHES[3] = {4, 8, 32};
+    int bb = 16;
+    for (int q = 0; q < 3; q++) {
+        c->dag_batch = BATCHES[q];
+        float ms = 0.f;
+        LEM_TRY(time_one(&ms));
+        c->tune_ms[11 + q] = ms;
+        if (ms < best) { best = ms; bb = BATCHES[q]; }
+    }
+    c->dag_batch = bb;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return LEMHIP_OK;
+}
+
+int lem_dag_shape_count() { return 6; }
+void lem_dag_shape_dims(int id, int *tw, int *th)
+{
+    switch (id) {
+#define X(i, a, b) case i: *tw = a; *th = b; return;
+    DAG_SHAPES(X)
+#undef X
+    default: *tw = 0; *th = 0;
+    }
 }
 
 template int lem_fa_dag<double>(lemhip_ctx *);

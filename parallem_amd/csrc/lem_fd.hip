@@ -102,7 +102,8 @@ template <typename TS>
 __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, const double *__restrict__ z,
                                                   const uint8_t *__restrict__ mask, TS *__restrict__ slopes,
                                                   long plane, uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
-                                                  int *__restrict__ cand, int *__restrict__ ncand)
+                                                  uint8_t *__restrict__ eqb, int *__restrict__ cand,
+                                                  int *__restrict__ ncand)
 {
     __shared__ int s_wave[4];
     __shared__ int s_base;
@@ -114,6 +115,7 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
         long self = (long)irow * W + icol;
         bool active = inside && mask[self];
         int aspect = 0, aspectsfd = 0;
+        unsigned eq = 0;
         if (active) {
             double zc = z[self];
             double smax = 0.0;
@@ -133,124 +135,217 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
                         aspect += kCODE[d];
                     }
                     if (zc < zn) up++;
-                    if (zc == zn) flat++;
+                    if (zc == zn) { flat++; eq |= 1u << d; }
                 }
             }
             if (up > 7) { aspect = 0; aspectsfd = 0; }
             if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
             if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
         }
-        if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; }
+        if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; eqb[self] = (uint8_t)eq; }
         int slot = block_append(ncand, active && aspect == 0 && aspectsfd == 0, s_wave, &s_base);
         if (slot >= 0) cand[slot] = (int)self;
     }
 }
 
 // ------------------------------------------------------ A3/A4: plateaus
+//
+// route_plateausMFD (newflow.cu:264-350) in synchronous rounds (D1): in round
+// r a candidate reads its equal-height neighbours' round r-1 distances; each
+// strict improvement in the scan (order N..NW) marks that slope and takes its
+// direction.  The rounds are temporally blocked: one launch advances K rounds
+// for a tile of PTW x PTH cells, loading the tile plus a K-cell halo into LDS
+// and recomputing the halo redundantly; after round k only cells at least k
+// cells from the halo's outer edge are current, so after K rounds the tile
+// itself is exact.  A cell can change in round r only if an equal-height
+// neighbour changed in round r-1, so a tile needs the next launch only when it
+// or one of its 8 neighbour tiles changed in the last round of this launch
+// (K <= tile width and height); tiles without candidates never change.
 
 __global__ void k_sp_init(long N, float *sp)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) sp[i] = 0.0f;
 }
 
-// candidates start at 2e7 (newflow.cu:252); the first round examines all of them
-__global__ void k_sp_seed(const int *cand, int n, float *sp, int *front)
+// candidates start at 2e7 (newflow.cu:252); non-candidates keep 0 and never change
+__global__ void k_sp_seed(int W, const int *cand, int n, float *sp, int ntx, int *tile_has)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) { sp[cand[k]] = 20000000.0f; front[k] = cand[k]; }
+    if (k >= n) return;
+    int i = cand[k];
+    sp[i] = 20000000.0f;
+    int t = ((i / W) / PTH) * ntx + (i % W) / PTW;
+    if (tile_has[t] == 0) tile_has[t] = 1;
 }
 
-// Counters of the frontier machinery, indexed by round: frontier F[r % 2]
-// (count cnt[r % 2]), update list U (count cnt[2 + r % 2]).
+// Launch bookkeeping: launch L visits list L % 3 (count cnt[L % 3]); a tile
+// still moving at its end queues itself and its neighbours with candidates in
+// list (L+1) % 3, each once (stamp[tile] holds the last launch it was queued
+// for), and launch L clears cnt[(L+2) % 3], last read by the publish step of
+// launch L-1.
 struct PlatCtl {
-    int cnt[4];
-    int changed_rounds;
-    int flag;
+    int cnt[3];
+    int pad0[29];
+    int last_change;   // last round (absolute number) in which any cell changed
+    int pad1[31];
 };
 
-// route_plateausMFD (newflow.cu:264-350), one synchronous round (D1), part 1:
-// cells of the frontier read only round r-1 distances (nothing writes sp in
-// this kernel); each strict improvement in the scan marks that slope and
-// takes its direction; improved cells are queued with their new distance.
-// Only a cell with an equal-height neighbour that changed in round r-1 can
-// improve in round r, so the frontier holds exactly those cells.
-template <typename TS>
-__global__ void __launch_bounds__(256) k_plat_scan(int W, int H, int r, const int *__restrict__ front,
-                                                   const double *__restrict__ z, const float *__restrict__ sp,
-                                                   uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
-                                                   TS *__restrict__ slopes, long plane, int *__restrict__ U,
-                                                   float *__restrict__ Uv, PlatCtl *ctl)
+__global__ void k_plat_first_list(int ntiles, const int *tile_has, int *list, PlatCtl *ctl, int slot)
 {
-    int n = ctl->cnt[r & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(r + 1) & 1] = 0;  // next frontier, filled by part 2
-    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {
-        int k = k0 + threadIdx.x;
-        bool changed = false;
-        float min_distance = 0.f;
-        long self = 0;
-        if (k < n) {
-            self = front[k];
-            int irow = (int)(self / W), icol = (int)(self % W);
-            min_distance = sp[self];
-            double zc = z[self];
-            int code = -1;
-            for (int d = 0; d < 8; d++) {
-                int cx = icol + kDX[d], cy = irow + kDY[d];
-                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
-                    float dc = (d & 1) ? 1.41f : 1.0f;
-                    long nb = (long)cy * W + cx;
-                    float dis = sp[nb] + dc;
-                    if (zc == z[nb] && min_distance > dis) {
-                        min_distance = dis;
-                        changed = true;
-                        code = kCODE[d];
-                        if (slopes) slopes[d * plane + self] = (TS)0.0001;
-                    }
-                }
-            }
-            if (changed) { fd[self] = (uint8_t)code; SFD[self] = (uint8_t)code; }
-        }
-        int slot = wave_append(&ctl->cnt[2 + (r & 1)], changed);
-        if (slot >= 0) { U[slot] = (int)self; Uv[slot] = min_distance; }
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    for (int t0 = blockIdx.x * blockDim.x; t0 < ntiles; t0 += gridDim.x * blockDim.x) {
+        int t = t0 + threadIdx.x;
+        bool have = t < ntiles && tile_has[t];
+        int at = block_append(&ctl->cnt[slot], have, s_wave, &s_base);
+        if (have) list[at] = t;
     }
 }
 
-// part 2: publish the round's new distances and queue, once, every
-// equal-height candidate neighbour of a changed cell for round r+1.  A cell is
-// a candidate exactly when its distance is nonzero, which no update changes,
-// so the test does not depend on the order of the writes in this kernel.
-__global__ void __launch_bounds__(256) k_plat_next(int W, int H, int r, const int *__restrict__ U,
-                                                   const float *__restrict__ Uv, const double *__restrict__ z,
-                                                   float *__restrict__ sp, int *__restrict__ stamp,
-                                                   int *__restrict__ front_next, PlatCtl *ctl)
+template <int K, typename TS>
+__global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, int r0, const int *__restrict__ act,
+                                                    int *__restrict__ next, const float *__restrict__ sp_in,
+                                                    float *__restrict__ sp_out, const uint8_t *__restrict__ eqb,
+                                                    uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
+                                                    TS *__restrict__ slopes, long plane,
+                                                    const int *__restrict__ tile_has, int *__restrict__ tstamp,
+                                                    PlatCtl *ctl)
 {
-    int m = ctl->cnt[2 + (r & 1)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->cnt[2 + ((r + 1) & 1)] = 0;
-        if (m > 0) { ctl->changed_rounds++; ctl->flag = 1; }
-    }
-    for (int k0 = blockIdx.x * blockDim.x; k0 < m; k0 += gridDim.x * blockDim.x) {
-        int k = k0 + threadIdx.x;
-        int picks[8];
-        int np = 0;
-        if (k < m) {
-            long self = U[k];
-            sp[self] = Uv[k];
-            int irow = (int)(self / W), icol = (int)(self % W);
-            double zc = z[self];
-            for (int d = 0; d < 8; d++) {
-                int cx = icol + kDX[d], cy = irow + kDY[d];
-                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
-                    long nb = (long)cy * W + cx;
-                    if (z[nb] == zc && sp[nb] != 0.0f && atomicExch(&stamp[nb], r + 1) != r + 1)
-                        picks[np++] = (int)nb;
-                }
+    constexpr int RW = PTW + 2 * K, RH = PTH + 2 * K, RN = RW * RH, NI = PTW * PTH;
+    constexpr int PER = (RN + 255) / 256, PERI = (NI + 255) / 256;
+    static_assert(PER <= 32, "region too large for the per-thread cell mask");
+    __shared__ float A[RN], B[RN];
+    __shared__ uint8_t E[RN];
+    __shared__ uint8_t code[NI], marks[NI];
+    __shared__ int last;
+    const int n = ctl->cnt[launch % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(launch + 2) % 3] = 0;
+    const int ntx = (W + PTW - 1) / PTW, nty = (H + PTH - 1) / PTH;
+    for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
+        const int tile = act[q0];
+        const int tx0 = (tile % ntx) * PTW, ty0 = (tile / ntx) * PTH;
+        __syncthreads();
+        if (threadIdx.x == 0) last = 0;
+        // load the region; both buffers hold every cell, so cells that cannot
+        // change (no candidate, or no equal-height neighbour) are never rewritten
+        unsigned own = 0;
+        float v[PER];
+        uint8_t e[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int idx = threadIdx.x + q * 256;
+            int gy = ty0 - K + idx / RW, gx = tx0 - K + idx % RW;
+            v[q] = 0.f;
+            e[q] = 0;
+            if (idx < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) {
+                long i = (long)gy * W + gx;
+                v[q] = sp_in[i];
+                e[q] = eqb[i];
             }
         }
-        for (int q = 0; q < 8; q++) {
-            bool have = q < np;
-            int slot = wave_append(&ctl->cnt[(r + 1) & 1], have);
-            if (have) front_next[slot] = picks[q];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int idx = threadIdx.x + q * 256;
+            if (idx < RN) {
+                A[idx] = v[q];
+                B[idx] = v[q];
+                E[idx] = e[q];
+                if (v[q] != 0.f && e[q] != 0) own |= 1u << q;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PERI; q++) {
+            int li = threadIdx.x + q * 256;
+            if (li < NI) { code[li] = 0; marks[li] = 0; }
+        }
+        __syncthreads();
+        float *cur = A, *nxt = B;
+        for (int k = 1; k <= K; k++) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                if (!((own >> q) & 1)) continue;
+                int idx = threadIdx.x + q * 256;
+                int rr = idx / RW, cc = idx % RW;
+                if (rr < k || rr >= RH - k || cc < k || cc >= RW - k) continue;
+                float md = cur[idx];
+                unsigned ee = e[q];
+                int cd = 0;
+                unsigned mk = 0;
+#pragma unroll
+                for (int d = 0; d < 8; d++) {
+                    if (!((ee >> d) & 1)) continue;
+                    float dc = (d & 1) ? 1.41f : 1.0f;
+                    float dis = cur[idx + kDY[d] * RW + kDX[d]] + dc;
+                    if (md > dis) { md = dis; cd = kCODE[d]; mk |= 1u << d; }
+                }
+                nxt[idx] = md;
+                if (cd && rr >= K && rr < K + PTH && cc >= K && cc < K + PTW) {
+                    int li = (rr - K) * PTW + (cc - K);
+                    code[li] = (uint8_t)cd;
+                    marks[li] |= (uint8_t)mk;
+                    atomicMax(&last, k);
+                }
+            }
+            __syncthreads();
+            float *t = cur; cur = nxt; nxt = t;
+        }
+        // write back the tile: distances, directions of changed cells, slope marks (pass 1 only)
+#pragma unroll
+        for (int q = 0; q < PERI; q++) {
+            int li = threadIdx.x + q * 256;
+            int gy = ty0 + li / PTW, gx = tx0 + li % PTW;
+            if (li >= NI || gy >= H || gx >= W) continue;
+            long i = (long)gy * W + gx;
+            sp_out[i] = cur[(li / PTW + K) * RW + (li % PTW + K)];
+            uint8_t cd = code[li];
+            if (cd) { fd[i] = cd; SFD[i] = cd; }
+            if (slopes && marks[li]) {
+                unsigned mk = marks[li];
+#pragma unroll
+                for (int d = 0; d < 8; d++)
+                    if ((mk >> d) & 1) slopes[d * plane + i] = (TS)0.0001;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && last > 0) atomicMax(&ctl->last_change, r0 + last);
+        // still moving at the end of the launch: this tile and its neighbours go on
+        if (last == K && threadIdx.x < 64) {   // wave 0 appends its picks with one atomic
+            int t2 = -1;
+            if (threadIdx.x < 9) {
+                int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+                if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
+                    t2 = ty * ntx + tx;
+                    if (!tile_has[t2] || atomicExch(&tstamp[t2], launch + 1) == launch + 1) t2 = -1;
+                }
+            }
+            unsigned long long m = __ballot(t2 >= 0);
+            int base = 0;
+            if (threadIdx.x == 0 && m) base = atomicAdd(&ctl->cnt[(launch + 1) % 3], __popcll(m));
+            base = __shfl(base, 0);
+            if (t2 >= 0) next[base + __popcll(m & ((1ULL << threadIdx.x) - 1))] = t2;
+        }
+    }
+}
+
+// the launch's tiles publish their new distances for the next launch's halos
+__global__ void __launch_bounds__(256) k_plat_publish(int W, int H, int launch, const int *__restrict__ act,
+                                                      const float *__restrict__ sp_out, float *__restrict__ sp_in,
+                                                      const PlatCtl *ctl)
+{
+    const int n = ctl->cnt[launch % 3];
+    const int ntx = (W + PTW - 1) / PTW;
+    for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
+        const int tile = act[q0];
+        {
+        const int tx0 = (tile % ntx) * PTW, ty0 = (tile / ntx) * PTH;
+#pragma unroll
+        for (int q = 0; q < (PTW * PTH + 255) / 256; q++) {
+            int li = threadIdx.x + q * 256;
+            int gy = ty0 + li / PTW, gx = tx0 + li % PTW;
+            if (li >= PTW * PTH || gy >= H || gx >= W) continue;
+            long i = (long)gy * W + gx;
+            sp_in[i] = sp_out[i];
+        }
         }
     }
 }
@@ -643,11 +738,12 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
 {
     long N = c->N;
     int W = c->W, H = c->H;
+    uint8_t *eqb = c->dmask;   // equal-height neighbour bits; the accumulation passes reuse the buffer later
     LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
     dim3 blk(64, 4);
     dim3 grd((W + 63) / 64, (H + 15) / 16);
     prof_begin(c, PROF_FLOWDIRS);
-    k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, c->NP, c->fd, c->SFD,
+    k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, c->NP, c->fd, c->SFD, eqb,
                                           c->cand, &c->scal->ncand);
     prof_end(c, PROF_FLOWDIRS);
     LEM_LAUNCHED(c);
@@ -657,45 +753,55 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     *ncand_out = n;
     *rounds = 0;
     if (n == 0) return LEMHIP_OK;
-    // frontier machinery: sp0 = distances, cand = round-1 frontier copy source,
-    // ws/scan_tmp hold the two frontiers, z2-sized scratch is not touched here
-    float *sp = c->sp0;
-    int *F[2] = {c->pl_front0, c->pl_front1};
-    PlatCtl *ctl = reinterpret_cast<PlatCtl *>(c->pl_ctl);
-    int r0 = c->pl_round;
-    if (r0 > (1 << 29)) {  // stamps hold round numbers: restart them well before they wrap
-        LEM_CHECK(c, hipMemsetAsync(c->pl_stamp, 0, N * sizeof(int), c->s));
-        r0 = 0;
-    }
-    k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, sp);
-    k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(c->cand, n, sp, F[(r0 + 1) & 1]);
-    LEM_LAUNCHED(c);
-    c->hint[8] = 0; c->hint[9] = 0; c->hint[10] = 0; c->hint[11] = 0;
-    c->hint[8 + ((r0 + 1) & 1)] = n;
-    c->hint[12] = 0; c->hint[13] = 0;
-    LEM_CHECK(c, hipMemcpyAsync(ctl, &c->hint[8], sizeof(PlatCtl), hipMemcpyHostToDevice, c->s));
     prof_begin(c, PROF_PLATEAU);
-    int r = r0;
-    const int BATCH = 32;
-    const int G = 128;
-    for (int batch = 0;; ++batch) {
-        LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));
+    const int ntx = (W + PTW - 1) / PTW, nty = (H + PTH - 1) / PTH, ntiles = ntx * nty;
+    int *tile_has = c->pl_tiles, *tstamp = c->pl_tiles + ntiles;
+    int *L[3] = {c->pl_tiles + 2 * (long)ntiles, c->pl_tiles + 3 * (long)ntiles, c->pl_tiles + 4 * (long)ntiles};
+    PlatCtl *ctl = reinterpret_cast<PlatCtl *>(c->pl_ctl);
+    float *sp_in = c->sp0, *sp_out = c->sp1;
+    int launch = c->pl_launch;
+    if (launch > (1 << 29)) {  // tile stamps hold launch numbers: restart them well before they wrap
+        LEM_CHECK(c, hipMemsetAsync(tstamp, 0, ntiles * sizeof(int), c->s));
+        launch = 0;
+    }
+    LEM_CHECK(c, hipMemsetAsync(tile_has, 0, ntiles * sizeof(int), c->s));
+    LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(PlatCtl), c->s));
+    k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, sp_in);
+    k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, c->cand, n, sp_in, ntx, tile_has);
+    k_plat_first_list<<<grid1d(ntiles, 256, 1024), 256, 0, c->s>>>(ntiles, tile_has, L[launch % 3], ctl, launch % 3);
+    LEM_LAUNCHED(c);
+    const int G = ntiles < 1024 ? ntiles : 1024;
+    const int BATCH = 8;
+    int r = 0;  // rounds done in this pass
+    long guard = 0;
+    for (;;) {
         for (int q = 0; q < BATCH; q++) {
-            ++r;
-            // round 1 scans every candidate; later frontiers are thin wavefronts
-            int g = (r == r0 + 1) ? grid1d(n, 256, 4096) : G;
-            k_plat_scan<TS><<<g, 256, 0, c->s>>>(W, H, r, F[r & 1], zs, sp, c->fd, c->SFD, slopes, c->NP, c->pl_U,
-                                                 c->pl_Uv, ctl);
-            k_plat_next<<<g, 256, 0, c->s>>>(W, H, r, c->pl_U, c->pl_Uv, zs, sp, c->pl_stamp, F[(r + 1) & 1], ctl);
+            const int *a_cur = L[launch % 3];
+            int *a_next = L[(launch + 1) % 3];
+            switch (c->plat_k) {
+#define PLAT_LAUNCH(K)                                                                                               \
+    k_plat_block<K, TS><<<G, 256, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, eqb, c->fd, c->SFD, slopes, \
+                                             c->NP, tile_has, tstamp, ctl)
+            case 4: PLAT_LAUNCH(4); break;
+            case 16: PLAT_LAUNCH(16); break;
+            default: PLAT_LAUNCH(8); break;
+#undef PLAT_LAUNCH
+            }
+            k_plat_publish<<<G, 256, 0, c->s>>>(W, H, launch, a_cur, sp_out, sp_in, ctl);
+            ++launch;
+            r += c->plat_k;
         }
         LEM_LAUNCHED(c);
-        LEM_CHECK(c, hipMemcpyAsync(&c->hint[8], ctl, sizeof(PlatCtl), hipMemcpyDeviceToHost, c->s));
+        LEM_CHECK(c, hipMemcpyAsync(&c->hint[24], ctl, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s));
+        LEM_CHECK(c, hipMemcpyAsync(&c->hint[27], &ctl->last_change, sizeof(int), hipMemcpyDeviceToHost, c->s));
         LEM_CHECK(c, hipStreamSynchronize(c->s));
-        if (!c->hint[13]) break;
-        if ((long)batch > (long)n + 2) { c->err = "plateau routing did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        if (c->hint[24 + launch % 3] == 0) break;   // nothing queued for the next launch
+        // every launch with a moving tile changes at least one cell, and a
+        // cell's distance only decreases through a finite set of path lengths
+        if (++guard > (long)n + 2) { c->err = "plateau routing did not converge"; return LEMHIP_E_NOT_CONVERGED; }
     }
-    c->pl_round = r;
-    *rounds = c->hint[12];
+    c->pl_launch = launch;
+    *rounds = c->hint[27];
     prof_end(c, PROF_PLATEAU, *rounds + 1);
     return LEMHIP_OK;
 }
@@ -869,6 +975,36 @@ static int flooding(lemhip_ctx *c, int ncand1)
     return LEMHIP_OK;
 }
 
+// Times the plateau pass with every rounds-per-launch candidate on the same
+// surface (the pass is idempotent: it rebuilds directions, distances and marks
+// from the surface) and keeps the fastest.
+template <typename TS>
+static int tune_plateau(lemhip_ctx *c, const double *zs, TS *slopes)
+{
+    static const int KS[3] = {4, 8, 16};
+    hipEvent_t e0, e1;
+    LEM_CHECK(c, hipEventCreate(&e0));
+    LEM_CHECK(c, hipEventCreate(&e1));
+    float best = 1e30f;
+    int bestk = c->plat_k;
+    for (int q = 0; q < 3; q++) {
+        c->plat_k = KS[q];
+        int n = 0, r = 0;
+        LEM_CHECK(c, hipEventRecord(e0, c->s));
+        LEM_TRY(directions_and_plateaus<TS>(c, zs, slopes, &n, &r));
+        LEM_CHECK(c, hipEventRecord(e1, c->s));
+        LEM_CHECK(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        LEM_CHECK(c, hipEventElapsedTime(&ms, e0, e1));
+        c->tune_ms[8 + q] = ms;
+        if (ms < best) { best = ms; bestk = KS[q]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->plat_k = bestk;
+    return LEMHIP_OK;
+}
+
 template <typename TS>
 static int fd_phase_t(lemhip_ctx *c, int iter)
 {
@@ -902,6 +1038,7 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
         k_slots_sinkcount_fin<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->ddesc, c->nslot, c->dsc);
         prof_end(c, PROF_FLOOD);
         int n2 = 0, r2 = 0;
+        if (c->tune_pending) LEM_TRY(tune_plateau<TS>(c, c->z2, (TS *)nullptr));
         LEM_TRY(directions_and_plateaus<TS>(c, c->z2, (TS *)nullptr, &n2, &r2));
         c->h_rounds2 = r2;
     }

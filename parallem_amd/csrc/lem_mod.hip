@@ -560,6 +560,7 @@ template <typename T>
 static int fa_t(lemhip_ctx *c)
 {
     using FS = typename FaS<T>::type;
+    if (c->tune_pending) LEM_TRY(lem_tune_dag(c));
     prof_begin(c, PROF_FA);
     LEM_TRY(lem_fa_dag<T>(c));
     prof_end(c, PROF_FA, c->dag_rounds_last);

@@ -41,11 +41,21 @@ EXPORTS = [
     "lemhip_profile_enable", "lemhip_profile_read", "lemhip_create_atlas", "lemhip_slot_open",
     "lemhip_slot_init", "lemhip_slot_close", "lemhip_slot_rect", "lemhip_slot_upload", "lemhip_slot_download",
     "lemhip_slot_get_outlet", "lemhip_slot_set_outlet", "lemhip_slot_set_climate", "lemhip_slot_summary",
-    "lemhip_step_slots",
+    "lemhip_step_slots", "lemhip_get_tuning", "lemhip_set_tuning", "lemhip_autotune",
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
               "mod2", "reductions", "step", "fa_rounds", "sed_rounds"]
+
+
+class LemTuning(C.Structure):
+    _fields_ = [("dag_tile_w", C.c_int), ("dag_tile_h", C.c_int), ("dag_batch", C.c_int),
+                ("plateau_rounds_per_launch", C.c_int), ("tuned_ms", C.c_double * 16)]
+
+    def as_dict(self) -> dict:
+        return {"dag_tile": [self.dag_tile_w, self.dag_tile_h], "dag_batch": self.dag_batch,
+                "plateau_rounds_per_launch": self.plateau_rounds_per_launch,
+                "tuned_ms": [round(x, 3) for x in self.tuned_ms]}
 
 
 class LemError(RuntimeError):
@@ -138,6 +148,9 @@ def lib() -> C.CDLL:
     L.lemhip_slot_set_climate.argtypes = [vp, C.c_int, C.c_double, C.c_double]
     L.lemhip_slot_summary.argtypes = [vp, C.c_int, C.POINTER(LemSummary)]
     L.lemhip_step_slots.argtypes = [vp, vp]
+    L.lemhip_get_tuning.argtypes = [vp, C.POINTER(LemTuning)]
+    L.lemhip_set_tuning.argtypes = [vp, C.POINTER(LemTuning)]
+    L.lemhip_autotune.argtypes = [vp]
     _lib = L
     return L
 
@@ -198,6 +211,27 @@ class _Handle:
 
     def copy_to_last_climate(self) -> None:
         self._check(self.L.lemhip_copy_to_last_climate(self.h), "copy_to_last_climate")
+
+    def tuning(self) -> dict:
+        t = LemTuning()
+        self._check(self.L.lemhip_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, dag_tile=None, dag_batch=None, plateau_rounds_per_launch=None) -> None:
+        t = LemTuning()
+        self._check(self.L.lemhip_get_tuning(self.h, C.byref(t)), "get_tuning")
+        if dag_tile is not None:
+            t.dag_tile_w, t.dag_tile_h = dag_tile
+        if dag_batch is not None:
+            t.dag_batch = dag_batch
+        if plateau_rounds_per_launch is not None:
+            t.plateau_rounds_per_launch = plateau_rounds_per_launch
+        self._check(self.L.lemhip_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def autotune(self) -> None:
+        """The next step times every launch candidate on this context's data
+        and keeps the fastest (results do not depend on the choice)."""
+        self._check(self.L.lemhip_autotune(self.h), "autotune")
 
 
 class LemContext(_Handle):

@@ -1,0 +1,53 @@
+"""Launch tuning never changes results: every dependency-engine tile shape,
+launch batch and plateau rounds-per-launch choice, and an autotuned step,
+give bit-identical fields (FP64 form; the oracle comparison of the default
+choice is in test_gpu_parity.py)."""
+import numpy as np
+import pytest
+
+from parallem_amd import lemhip, synth
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["dem", "fa", "soilT", "stone", "depo", "Slopes", "fd", "SFD"]
+CHOICES = [dict(dag_tile=(32, 32)), dict(dag_tile=(16, 64), dag_batch=4), dict(dag_tile=(128, 8)),
+           dict(dag_tile=(64, 32), plateau_rounds_per_launch=4), dict(dag_tile=(32, 64), plateau_rounds_per_launch=16)]
+
+
+def _run(dem, flowtype, tuning=None, autotune=False, steps=3):
+    H, W = dem.z.shape
+    ctx = lemhip.LemContext(lemhip.default_params(W, H, flowtype), lemhip.FP64, 0)
+    ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+    if tuning:
+        ctx.set_tuning(**tuning)
+    for k in range(steps):
+        if autotune and k == 1:
+            ctx.autotune()
+        ctx.step(-1000 + k)
+    out = {f: ctx.download(f) for f in FIELDS}
+    out["tuning"] = ctx.tuning()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("flowtype", [0, 1])
+def test_every_tuning_gives_identical_results(flowtype):
+    dem = synth.config_dem(1, side=300, seed_offset=31)
+    ref = _run(dem, flowtype)
+    for t in CHOICES + [None]:
+        got = _run(dem, flowtype, tuning=t, autotune=t is None)
+        for f in FIELDS:
+            assert np.array_equal(got[f], ref[f], equal_nan=True), f"{t}: {f}"
+    tuned = got["tuning"]
+    assert tuned["plateau_rounds_per_launch"] in (4, 8, 16)
+    assert tuple(tuned["dag_tile"]) in {(64, 16), (32, 32), (16, 64), (128, 8), (64, 32), (32, 64)}
+    assert all(x > 0 for x in tuned["tuned_ms"][:6])
+
+
+def test_bad_tuning_is_rejected():
+    ctx = lemhip.LemContext(lemhip.default_params(32, 32), lemhip.FP64, 0)
+    with pytest.raises(lemhip.LemError):
+        ctx.set_tuning(dag_tile=(48, 48))
+    with pytest.raises(lemhip.LemError):
+        ctx.set_tuning(plateau_rounds_per_launch=5)
+    ctx.close()
