@@ -14,7 +14,10 @@ unclamped formula turns every cell below 400 m into NaN after one step):
 * plain_fp64 — one such DEM alone in the plain FP64 form (bit-exact with the
   CPU restatement): the reference's own arithmetic.
 
-A step is one full FD -> FA -> MOD timestep of every DEM.  Every step call
+Before timing, one warmup step runs the launch autotuner (tile shape and
+launch batch of the dependency engine, plateau rounds per launch), which
+never changes results.  A step is one full FD -> FA -> MOD timestep of every
+DEM.  Every step call
 returns after a stream synchronisation, so the wall clock around the K timed
 steps brackets the device work exactly.  cells counted = sum of W*H over DEMs.
 """
@@ -73,9 +76,11 @@ def kernel_bytes(kernel, t):
 
 # kernel-level timing slots (HIP events around exactly these launches)
 KERNEL_SLOTS = ("fa_rounds", "sed_rounds", "mod1", "mod2", "flowdirs")
-# rocprofv3 kernel names of those slots
-ROCPROF_NAMES = {"fa_rounds": "k_dag_round<64, 16, FaCell", "sed_rounds": "k_dag_round<64, 16, SedCell",
-                 "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs"}
+# rocprofv3 kernel names of those slots (the round kernels carry the tuned tile shape)
+def rocprof_name(slot, tuning):
+    tw, th = tuning["dag_tile"]
+    return {"fa_rounds": f"k_dag_round<{tw}, {th}, FaCell", "sed_rounds": f"k_dag_round<{tw}, {th}, SedCell",
+            "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs"}[slot]
 
 
 def dems(count):
@@ -83,8 +88,11 @@ def dems(count):
     return [synth.config_dem(2, seed_offset=k, side=SIDE) for k in range(count)]
 
 
-def timed(step, steps, warmup, handle):
-    for _ in range(warmup):
+def timed(step, steps, warmup, handle, autotune=True):
+    if autotune:
+        # the first warmup step times the launch candidates on this data and keeps the fastest
+        handle.autotune()
+    for _ in range(max(warmup, 1 if autotune else 0)):
         step()
     handle.profile(True)
     lat = []
@@ -97,22 +105,25 @@ def timed(step, steps, warmup, handle):
     return total, np.array(lat), handle.profile_read()
 
 
-def run_single(dem, precision, steps, warmup):
+def run_single(dem, precision, steps, warmup, tuning=None):
     from parallem_amd import lemhip
     ctx = lemhip.LemContext(lemhip.default_params(SIDE, SIDE, flowtype=0), precision, 0)
     ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+    if tuning:
+        ctx.set_tuning(**tuning)
     it = [-1000]
 
     def step():
         ctx.step(it[0])
         it[0] += 1
-    total, lat, prof = timed(step, steps, warmup, ctx)
+    total, lat, prof = timed(step, steps, warmup, ctx, autotune=tuning is None)
     summ = ctx.summary()
+    summ["tuning"] = ctx.tuning()
     ctx.close()
     return total, lat, prof, summ
 
 
-def run_batched(ds, precision, steps, warmup):
+def run_batched(ds, precision, steps, warmup, tuning=None):
     from parallem_amd import lemhip
     cols = int(np.ceil(np.sqrt(len(ds))))
     rows = int(np.ceil(len(ds) / cols))
@@ -120,8 +131,11 @@ def run_batched(ds, precision, steps, warmup):
     atlas = lemhip.LemAtlas(lemhip.default_params(aw, ah, flowtype=0), aw, ah, precision, 0)
     slots = [atlas.add(d.z, d.rain, soilT_mode="nonneg") for d in ds]
     assert all(s is not None for s in slots)
-    total, lat, prof = timed(atlas.step, steps, warmup, atlas)
+    if tuning:
+        atlas.set_tuning(**tuning)
+    total, lat, prof = timed(atlas.step, steps, warmup, atlas, autotune=tuning is None)
     summ = [atlas.summary(s) for s in slots]
+    summ[0]["tuning"] = atlas.tuning()
     atlas.close()
     return total, lat, prof, summ, (aw, ah)
 
@@ -130,7 +144,7 @@ def phase_ms(prof, steps):
     return {k: round(v[0] / steps, 3) for k, v in prof.items()}
 
 
-def roofline(prof, steps, cells, t):
+def roofline(prof, steps, cells, t, tuning):
     """Dominant kernel by live HIP-event time; achieved = algorithmic bytes
     per launch / average launch duration (= bytes of all its launches / their
     total time)."""
@@ -142,12 +156,32 @@ def roofline(prof, steps, cells, t):
     passes = steps if dom in ("fa_rounds", "sed_rounds") else launches
     total_bytes = b * cells * passes
     ach = total_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
-    return {"kernel": ROCPROF_NAMES[dom], "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+    return {"kernel": rocprof_name(dom, tuning), "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "achieved": ach, "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None,
             "bytes_per_cell": b, "launches_per_step": launches / steps,
             "avg_launch_us": 1e3 * ms / launches if launches else None,
             "bytes_per_launch": total_bytes / launches if launches else None,
             "ms_per_step": ms / steps}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batched_4x4096_fp32_traffic.json")
+
+
+def measured_traffic(kernel, tuning):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs of the same workload and tuning (calibrated
+    against a copy kernel of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if list(t.get("tuning", {}).get("dag_tile", [])) != list(tuning["dag_tile"]):
+        return None
+    for name, v in t.get("kernels", {}).items():
+        if kernel in name:
+            return v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(side, threads):
@@ -171,6 +205,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the single-DEM forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-side", type=int, default=2048)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB three times (a known-size kernel for counter calibration)")
+    ap.add_argument("--tuning", default=None,
+                    help="tw,th,batch,k: fixed launch tuning instead of the autotuner (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
@@ -178,28 +216,41 @@ def main():
         sys.exit(2)
 
     from parallem_amd import lemhip
+    if args.copy_calibration > 0:
+        import torch
+        a = torch.ones(args.copy_calibration * 262144, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    fixed = None
+    if args.tuning:
+        tw, th, bt, k = (int(x) for x in args.tuning.split(","))
+        fixed = dict(dag_tile=(tw, th), dag_batch=bt, plateau_rounds_per_launch=k)
     print(f"[bench] generating {args.batch} DEMs of {SIDE}x{SIDE}", file=sys.stderr, flush=True)
     ds = dems(max(args.batch, 1))
     cells_b = args.batch * SIDE * SIDE
 
     print("[bench] batched FP32+Kahan form", file=sys.stderr, flush=True)
-    tb, latb, profb, summb, atlas_dims = run_batched(ds[:args.batch], lemhip.FP32_KAHAN, args.steps, args.warmup)
+    tb, latb, profb, summb, atlas_dims = run_batched(ds[:args.batch], lemhip.FP32_KAHAN, args.steps, args.warmup,
+                                                     tuning=fixed)
     value = cells_b * args.steps / tb
 
     single32 = plain = None
     if not args.no_plain:
         print("[bench] single-DEM FP32+Kahan form", file=sys.stderr, flush=True)
-        t32, lat32, prof32, s32 = run_single(ds[0], lemhip.FP32_KAHAN, args.steps, args.warmup)
+        t32, lat32, prof32, s32 = run_single(ds[0], lemhip.FP32_KAHAN, args.steps, args.warmup, tuning=fixed)
         single32 = {"value": SIDE * SIDE * args.steps / t32, "ms_per_step": 1e3 * t32 / args.steps,
                     "p50_ms": float(np.percentile(lat32, 50) * 1e3), "p99_ms": float(np.percentile(lat32, 99) * 1e3),
-                    "phase_ms_per_step": phase_ms(prof32, args.steps)}
+                    "phase_ms_per_step": phase_ms(prof32, args.steps), "tuning": s32["tuning"]}
         print("[bench] single-DEM plain FP64 form", file=sys.stderr, flush=True)
-        t64, lat64, prof64, s64 = run_single(ds[0], lemhip.FP64, args.steps, args.warmup)
+        t64, lat64, prof64, s64 = run_single(ds[0], lemhip.FP64, args.steps, args.warmup, tuning=fixed)
         plain = {"value": SIDE * SIDE * args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps,
                  "p50_ms": float(np.percentile(lat64, 50) * 1e3), "p99_ms": float(np.percentile(lat64, 99) * 1e3),
                  "phase_ms_per_step": phase_ms(prof64, args.steps),
                  "step_roofline_frac": SIDE * SIDE * args.steps / t64 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9),
-                 "roofline": roofline(prof64, args.steps, SIDE * SIDE, 8),
+                 "roofline": roofline(prof64, args.steps, SIDE * SIDE, 8, s64["tuning"]), "tuning": s64["tuning"],
                  "last_step": {k: s64[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2",
                                                     "fa_uncomputed", "sed_uncomputed")}}
 
@@ -213,6 +264,10 @@ def main():
                "sample": f"one FP64 timestep of a {args.cpu_side}x{args.cpu_side} DEM from the same generator "
                          f"(C restatement, OpenMP), {dt:.1f} s", "host": platform.processor() or platform.machine()}
 
+    roof = roofline(profb, args.steps, atlas_dims[0] * atlas_dims[1], 4, summb[0]["tuning"])
+    roof["traffic"] = measured_traffic(roof["kernel"], summb[0]["tuning"])
+    if roof["traffic"] is not None:
+        roof["traffic_note"] = "HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * tb / args.steps, "higher_is_better": True,
@@ -225,12 +280,13 @@ def main():
         "p50_ms": float(np.percentile(latb, 50) * 1e3), "p99_ms": float(np.percentile(latb, 99) * 1e3),
         "step_roofline_frac": value * B_ALG["fp32"] / (HBM_PEAK_GBS * 1e9),
         "phase_ms_per_step": phase_ms(profb, args.steps),
-        "roofline": roofline(profb, args.steps, atlas_dims[0] * atlas_dims[1], 4),
+        "roofline": roof,
         "single_fp32": single32,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
         "device": lemhip.device_info(0),
+        "tuning": summb[0]["tuning"],
         "last_step": [{k: s[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2", "fa_uncomputed",
                                          "sed_uncomputed")} for s in summb[:1]],
     }

@@ -188,6 +188,11 @@ static void free_all(lemhip_ctx *c)
                     c->dstage};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    for (int q = 0; q < 2; q++) {
+        if (c->elo2[q]) (void)hipFree(c->elo2[q]);
+        if (c->ehi2[q]) (void)hipFree(c->ehi2[q]);
+        if (c->ew2[q]) (void)hipFree(c->ew2[q]);
+    }
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hdsc) (void)hipHostFree(c->hdsc);
     if (c->hflags) (void)hipHostFree(c->hflags);

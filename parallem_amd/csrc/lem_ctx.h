@@ -101,6 +101,8 @@ struct lemhip_ctx {
     long hcap = 0;
     uint32_t *elo = nullptr, *ehi = nullptr;
     unsigned long long *ew = nullptr;                        // compacted edges (weight encoded)
+    uint32_t *elo2[2] = {nullptr, nullptr}, *ehi2[2] = {nullptr, nullptr};   // edges still able to lower a value
+    unsigned long long *ew2[2] = {nullptr, nullptr};
     long ecap = 0;
     unsigned long long *rmm = nullptr;                       // minimax per label
     uint32_t *comp = nullptr;                                // component of labels cut off from label 0

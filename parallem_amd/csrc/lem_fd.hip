@@ -608,6 +608,31 @@ __global__ void k_minimax_relax(int ne, const uint32_t *elo, const uint32_t *ehi
     if (cb < rb) { atomicMin(&r[b], cb); *flag = 1; }
 }
 
+// An edge with weight >= the values of both its labels can never lower
+// either (values only decrease and max(w, r) >= w); the relaxation drops such
+// edges between batches of passes and keeps iterating over the rest.
+__global__ void __launch_bounds__(256) k_edges_keep(int ne, const uint32_t *elo, const uint32_t *ehi,
+                                                    const unsigned long long *ew, const unsigned long long *r,
+                                                    uint32_t *lo2, uint32_t *hi2, unsigned long long *w2, int *count)
+{
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    for (int k0 = blockIdx.x * blockDim.x; k0 < ne; k0 += gridDim.x * blockDim.x) {
+        int k = k0 + threadIdx.x;
+        bool keep = false;
+        uint32_t a = 0, b = 0;
+        unsigned long long w = 0;
+        if (k < ne) {
+            a = elo[k];
+            b = ehi[k];
+            w = ew[k];
+            keep = w < r[a] || w < r[b];
+        }
+        int at = block_append(count, keep, s_wave, &s_base);
+        if (keep) { lo2[at] = a; hi2[at] = b; w2[at] = w; }
+    }
+}
+
 // Basins with no path to label 0 (only possible when NaN heights cut them
 // off): the reference's union-find never marks them done, merges them in
 // ascending order and leaves every member with the height of the component's
@@ -812,6 +837,11 @@ static int ensure_hash(lemhip_ctx *c, long want)
     while (cap < want) cap <<= 1;
     if (cap <= c->hcap) return LEMHIP_OK;
     if (c->hkeys) { (void)hipFree(c->hkeys); (void)hipFree(c->hvals); (void)hipFree(c->elo); (void)hipFree(c->ehi); (void)hipFree(c->ew); }
+    for (int q = 0; q < 2; q++) {
+        if (c->elo2[q]) { (void)hipFree(c->elo2[q]); (void)hipFree(c->ehi2[q]); (void)hipFree(c->ew2[q]); }
+        c->elo2[q] = c->ehi2[q] = nullptr;
+        c->ew2[q] = nullptr;
+    }
     c->hkeys = c->hvals = c->ew = nullptr;
     c->elo = c->ehi = nullptr;
     c->hcap = 0;
@@ -820,6 +850,11 @@ static int ensure_hash(lemhip_ctx *c, long want)
     LEM_CHECK(c, hipMalloc(&c->elo, cap * 4));
     LEM_CHECK(c, hipMalloc(&c->ehi, cap * 4));
     LEM_CHECK(c, hipMalloc(&c->ew, cap * 8));
+    for (int q = 0; q < 2; q++) {
+        LEM_CHECK(c, hipMalloc(&c->elo2[q], cap * 4));
+        LEM_CHECK(c, hipMalloc(&c->ehi2[q], cap * 4));
+        LEM_CHECK(c, hipMalloc(&c->ew2[q], cap * 8));
+    }
     c->hcap = cap;
     c->ecap = cap;
     return LEMHIP_OK;
@@ -952,17 +987,31 @@ static int flooding(lemhip_ctx *c, int ncand1)
     int ne = c->hscal->nedges;
     c->h_nedges = ne;
 
-    for (int it = 0;; it += 16) {
-        LEM_TRY(clear_flag(c));
-        if (ne > 0)
-            for (int q = 0; q < 16; q++)
-                k_minimax_relax<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->ew, c->rmm,
-                                                                            &c->scal->flag);
-        LEM_LAUNCHED(c);
-        int f = 0;
-        LEM_TRY(read_flag(c, &f));
-        if (!f) break;
-        if (it / 16 > nlab + 2) { c->err = "spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    {
+        // relaxation passes over a shrinking set of edges that can still lower a value
+        const uint32_t *lo = c->elo, *hi = c->ehi;
+        const unsigned long long *w = c->ew;
+        int na = ne, buf = 0;
+        for (int it = 0;; it += 16) {
+            LEM_TRY(clear_flag(c));
+            if (na > 0)
+                for (int q = 0; q < 16; q++)
+                    k_minimax_relax<<<grid1d(na, 256, 1 << 30), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, &c->scal->flag);
+            LEM_LAUNCHED(c);
+            uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
+            unsigned long long *w2 = c->ew2[buf];
+            LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
+            if (na > 0)
+                k_edges_keep<<<grid1d(na, 256, 8192), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, lo2, hi2, w2,
+                                                                      &c->scal->nedges);
+            LEM_LAUNCHED(c);
+            LEM_TRY(read_scalars(c));
+            if (!c->hscal->flag) break;
+            na = c->hscal->nedges;
+            lo = lo2; hi = hi2; w = w2;
+            buf ^= 1;
+            if (it / 16 > nlab + 2) { c->err = "spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        }
     }
     LEM_TRY(clear_flag(c));
     if (ne > 0) k_fb_unreached<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->rmm, &c->scal->flag);
