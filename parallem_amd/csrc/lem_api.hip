@@ -259,9 +259,9 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     LEM_TRY(A((void **)&c->sp0, N * 4));
     LEM_TRY(A((void **)&c->sp1, N * 4));
     long ptiles = (long)((W + PTW - 1) / PTW) * ((H + PTH - 1) / PTH);
-    LEM_TRY(A((void **)&c->pl_tiles, 5 * ptiles * 4));
+    LEM_TRY(A((void **)&c->pl_tiles, 6 * ptiles * 4));
     LEM_TRY(A(&c->pl_ctl, 256));
-    LEM_CHECK(c, hipMemset(c->pl_tiles, 0, 5 * ptiles * 4));
+    LEM_CHECK(c, hipMemset(c->pl_tiles, 0, 6 * ptiles * 4));
     LEM_TRY(A((void **)&c->cand, N * 4));
     LEM_TRY(A((void **)&c->ws, N * 4));
     LEM_TRY(A((void **)&c->scan_tmp, N * 4));

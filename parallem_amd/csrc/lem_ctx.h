@@ -87,8 +87,8 @@ struct lemhip_ctx {
     void *dz = nullptr, *wC = nullptr, *wP = nullptr;
 
     // ---- scratch ----
-    float *sp0 = nullptr, *sp1 = nullptr;   // plateau distances: published / written by the current launch
-    int *pl_tiles = nullptr;                 // per plateau tile: has-candidates, stamp, 3 rotating lists
+    float *sp0 = nullptr, *sp1 = nullptr;   // plateau distances, two buffers per tile
+    int *pl_tiles = nullptr;                 // per plateau tile: has-candidates, stamp, 3 rotating lists, buffer version
     void *pl_ctl = nullptr;
     int pl_launch = 0;
     int *cand = nullptr;       // plateau candidates (also holds pass-2 list)

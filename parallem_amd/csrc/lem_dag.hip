@@ -125,21 +125,24 @@ struct SedCell {
     double *soilT;
     double ddk, dck, dgk;
     struct Lds {
+        double sT[NC];
         T ero[NC], inci[NC], geli[NC], stone[NC];
         T hero[NH], hinci[NH], hgeli[NH], hstone[NH];
     };
     struct Reg {
         T e, in, g, s;
+        double sT;
     };
-    __device__ __forceinline__ Reg load(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i]}; }
+    __device__ __forceinline__ Reg load(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i], soilT[i]}; }
     __device__ __forceinline__ void store(Lds &L, int li, const Reg &r) const
     {
         L.ero[li] = r.e;
         L.inci[li] = r.in;
         L.geli[li] = r.g;
         L.stone[li] = r.s;
+        L.sT[li] = r.sT;
     }
-    __device__ __forceinline__ Reg load_halo(long i) const { return load(i); }
+    __device__ __forceinline__ Reg load_halo(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i], 0.0}; }
     __device__ __forceinline__ void store_halo(Lds &L, int h, const Reg &r) const
     {
         L.hero[h] = r.e;
@@ -178,7 +181,7 @@ struct SedCell {
         T er = L.ero[li], ic = L.inci[li], ge = L.geli[li];
         T dp = ((accum_d + er) * (T)ddk) + ((accum_c + ic) * (T)dck) + ((accum_g + ge) * (T)dgk);
         depo[i] = dp;
-        double sT = soilT[i];
+        double sT = L.sT[li];
         double sedthick = sT + (double)dp;
         if (sedthick != 0.0) {
             double old = sT / sedthick;

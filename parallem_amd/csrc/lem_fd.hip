@@ -98,6 +98,11 @@ __device__ __forceinline__ void slot_hist_flush(int *hist, DemScal *ds, int off,
 // Slopes are written only for downhill neighbours (sticky state, D7); pass 2
 // passes slopes == nullptr.  Cells without a downhill neighbour are appended
 // to the plateau candidate list (the cells whose initial distance is 2e7).
+// A 64 x 16 tile per 64 x 4 block: the tile's surface plus a one-cell ring is
+// staged in LDS with every load in flight at once, then each thread evaluates
+// four rows from LDS.  Off-grid ring cells hold NaN: every comparison with it
+// is false, so such a neighbour is neither downhill, up nor flat, exactly as
+// a skipped one.
 template <typename TS>
 __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, const double *__restrict__ z,
                                                   const uint8_t *__restrict__ mask, TS *__restrict__ slopes,
@@ -105,38 +110,60 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
                                                   uint8_t *__restrict__ eqb, int *__restrict__ cand,
                                                   int *__restrict__ ncand)
 {
+    constexpr int RW = 66, RH = 18, RN = RW * RH, PER = (RN + 255) / 256;
+    __shared__ double zs[RN];
     __shared__ int s_wave[4];
     __shared__ int s_base;
-    int icol = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 16;
+    double v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int k = tid + q * 256;
+        int gy = y0 - 1 + k / RW, gx = x0 - 1 + k % RW;
+        v[q] = lm_nan();
+        if (k < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) v[q] = z[(long)gy * W + gx];
+    }
+    uint8_t mk[4];
+#pragma unroll
+    for (int rg = 0; rg < 4; rg++) {
+        int irow = y0 + rg * 4 + threadIdx.y, icol = x0 + threadIdx.x;
+        mk[rg] = (icol < W && irow < H) ? mask[(long)irow * W + icol] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int k = tid + q * 256;
+        if (k < RN) zs[k] = v[q];
+    }
+    __syncthreads();
+    const int icol = x0 + threadIdx.x;
 #pragma unroll 1
     for (int rg = 0; rg < 4; rg++) {
-        int irow = blockIdx.y * 16 + rg * 4 + threadIdx.y;
+        int lr = rg * 4 + threadIdx.y;
+        int irow = y0 + lr;
         bool inside = icol < W && irow < H;
         long self = (long)irow * W + icol;
-        bool active = inside && mask[self];
+        bool active = inside && mk[rg];
         int aspect = 0, aspectsfd = 0;
         unsigned eq = 0;
         if (active) {
-            double zc = z[self];
+            const int c0 = (lr + 1) * RW + (threadIdx.x + 1);
+            double zc = zs[c0];
             double smax = 0.0;
             int up = 0, flat = 0;
 #pragma unroll
             for (int d = 0; d < 8; d++) {
-                int cx = icol + kDX[d], cy = irow + kDY[d];
-                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
-                    float dc = (d & 1) ? 1.41f : 1.0f;
-                    float den = (float)cell_size * dc;
-                    double zn = z[(long)cy * W + cx];
-                    double stemp = (zc - zn) / (double)den;
-                    if (zn == -9999) stemp = -0.0000001;
-                    if ((zc > zn) && (zn != -9999)) {
-                        if (slopes) slopes[d * plane + self] = (TS)stemp;
-                        if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
-                        aspect += kCODE[d];
-                    }
-                    if (zc < zn) up++;
-                    if (zc == zn) { flat++; eq |= 1u << d; }
+                float dc = (d & 1) ? 1.41f : 1.0f;
+                float den = (float)cell_size * dc;
+                double zn = zs[c0 + kDY[d] * RW + kDX[d]];
+                double stemp = (zc - zn) / (double)den;
+                if ((zc > zn) && (zn != -9999)) {
+                    if (slopes) slopes[d * plane + self] = (TS)stemp;
+                    if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
+                    aspect += kCODE[d];
                 }
+                if (zc < zn) up++;
+                if (zc == zn) { flat++; eq |= 1u << d; }
             }
             if (up > 7) { aspect = 0; aspectsfd = 0; }
             if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
@@ -202,10 +229,28 @@ __global__ void k_plat_first_list(int ntiles, const int *tile_has, int *list, Pl
     }
 }
 
+// Distances live in two buffers per tile: tver[tile] = 2 * (launch that last
+// wrote the tile) + (buffer it wrote).  A launch reads every tile from the
+// buffer holding its state as of the previous launch and a visited tile
+// writes the other one, so no launch reads what another visit of the same
+// launch writes, and no copy-back is needed.
+__device__ __forceinline__ int plat_buf_as_of(int tv, int launch)
+{
+    int b = tv & 1;
+    return (tv >> 1) == launch ? 1 - b : b;
+}
+
+__global__ void k_plat_tver_init(int ntiles, int *tver)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) tver[t] = -2;   // buffer 0, written before any launch
+}
+
 template <int K, typename TS>
 __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, int r0, const int *__restrict__ act,
-                                                    int *__restrict__ next, const float *__restrict__ sp_in,
-                                                    float *__restrict__ sp_out, const uint8_t *__restrict__ eqb,
+                                                    int *__restrict__ next, float *__restrict__ sp0,
+                                                    float *__restrict__ sp1, int *__restrict__ tver,
+                                                    const uint8_t *__restrict__ eqb,
                                                     uint8_t *__restrict__ fd, uint8_t *__restrict__ SFD,
                                                     TS *__restrict__ slopes, long plane,
                                                     const int *__restrict__ tile_has, int *__restrict__ tstamp,
@@ -218,6 +263,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
     __shared__ uint8_t E[RN];
     __shared__ uint8_t code[NI], marks[NI];
     __shared__ int last;
+    __shared__ int s_buf[9];
     const int n = ctl->cnt[launch % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(launch + 2) % 3] = 0;
     const int ntx = (W + PTW - 1) / PTW, nty = (H + PTH - 1) / PTH;
@@ -226,7 +272,12 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         const int tx0 = (tile % ntx) * PTW, ty0 = (tile / ntx) * PTH;
         __syncthreads();
         if (threadIdx.x == 0) last = 0;
-        // load the region; both buffers hold every cell, so cells that cannot
+        if (threadIdx.x < 9) {
+            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+            s_buf[threadIdx.x] = (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) ? plat_buf_as_of(tver[ty * ntx + tx], launch) : 0;
+        }
+        __syncthreads();
+        // load the region; both LDS buffers hold every cell, so cells that cannot
         // change (no candidate, or no equal-height neighbour) are never rewritten
         unsigned own = 0;
         float v[PER];
@@ -239,7 +290,8 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
             e[q] = 0;
             if (idx < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) {
                 long i = (long)gy * W + gx;
-                v[q] = sp_in[i];
+                int ny = gy < ty0 ? 0 : (gy >= ty0 + PTH ? 2 : 1), nx = gx < tx0 ? 0 : (gx >= tx0 + PTW ? 2 : 1);
+                v[q] = (s_buf[ny * 3 + nx] ? sp1 : sp0)[i];
                 e[q] = eqb[i];
             }
         }
@@ -289,7 +341,9 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
             __syncthreads();
             float *t = cur; cur = nxt; nxt = t;
         }
-        // write back the tile: distances, directions of changed cells, slope marks (pass 1 only)
+        // write back the tile: distances (into its other buffer), directions of
+        // changed cells, slope marks (pass 1 only)
+        float *sp_out = s_buf[4] ? sp0 : sp1;
 #pragma unroll
         for (int q = 0; q < PERI; q++) {
             int li = threadIdx.x + q * 256;
@@ -307,7 +361,10 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0 && last > 0) atomicMax(&ctl->last_change, r0 + last);
+        if (threadIdx.x == 0) {
+            tver[tile] = (launch << 1) | (s_buf[4] ? 0 : 1);
+            if (last > 0) atomicMax(&ctl->last_change, r0 + last);
+        }
         // still moving at the end of the launch: this tile and its neighbours go on
         if (last == K && threadIdx.x < 64) {   // wave 0 appends its picks with one atomic
             int t2 = -1;
@@ -323,29 +380,6 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
             if (threadIdx.x == 0 && m) base = atomicAdd(&ctl->cnt[(launch + 1) % 3], __popcll(m));
             base = __shfl(base, 0);
             if (t2 >= 0) next[base + __popcll(m & ((1ULL << threadIdx.x) - 1))] = t2;
-        }
-    }
-}
-
-// the launch's tiles publish their new distances for the next launch's halos
-__global__ void __launch_bounds__(256) k_plat_publish(int W, int H, int launch, const int *__restrict__ act,
-                                                      const float *__restrict__ sp_out, float *__restrict__ sp_in,
-                                                      const PlatCtl *ctl)
-{
-    const int n = ctl->cnt[launch % 3];
-    const int ntx = (W + PTW - 1) / PTW;
-    for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
-        const int tile = act[q0];
-        {
-        const int tx0 = (tile % ntx) * PTW, ty0 = (tile / ntx) * PTH;
-#pragma unroll
-        for (int q = 0; q < (PTW * PTH + 255) / 256; q++) {
-            int li = threadIdx.x + q * 256;
-            int gy = ty0 + li / PTW, gx = tx0 + li % PTW;
-            if (li >= PTW * PTH || gy >= H || gx >= W) continue;
-            long i = (long)gy * W + gx;
-            sp_in[i] = sp_out[i];
-        }
         }
     }
 }
@@ -477,45 +511,74 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 // fixed table (D4): every mask cell c and in-grid neighbour n with different
 // labels and z_c >= z_n contributes edge {ws_c, ws_n} at height z_c; the
 // table keeps the minimum per unordered pair (order-free, exact).
+__device__ __forceinline__ void edge_insert_global(unsigned long long key, unsigned long long wenc,
+                                                   unsigned long long *keys, unsigned long long *vals, long cap,
+                                                   int *overflow)
+{
+    unsigned long long m = (unsigned long long)(cap - 1);
+    unsigned long long h = mix64(key) & m;
+    long probes = 0;
+    while (true) {
+        unsigned long long prev = atomicCAS(&keys[h], HASH_EMPTY, key);
+        if (prev == HASH_EMPTY || prev == key) { atomicMin(&vals[h], wenc); break; }
+        h = (h + 1) & m;
+        if (++probes >= cap) { *overflow = 1; break; }
+    }
+}
+
+// The cells of a block share most of their basin pairs (a boundary runs
+// through many cells): pairs are first merged in an LDS table (minimum weight
+// per pair, exact), and each distinct pair of the block goes to the global
+// table once.  Pairs that do not find an LDS slot go to the global table
+// directly; the minimum is the same either way.
+#define EDGE_LDS 1024
 __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__restrict__ mask, const int *__restrict__ ws,
                                                const double *__restrict__ z, const uint8_t *__restrict__ demid,
                                                unsigned long long *keys, unsigned long long *vals, long cap,
                                                int *overflow)
 {
+    __shared__ unsigned long long lk[EDGE_LDS], lv[EDGE_LDS];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int q = tid; q < EDGE_LDS; q += 256) { lk[q] = HASH_EMPTY; lv[q] = HASH_EMPTY; }
+    __syncthreads();
     int icol = blockIdx.x * blockDim.x + threadIdx.x;
     int irow = blockIdx.y * blockDim.y + threadIdx.y;
-    if (icol >= W || irow >= H) return;
     long c = (long)irow * W + icol;
-    if (!mask[c]) return;
-    int me = ws[c];
-    double zc = z[c];
-    int seen[8];
-    int ns = 0;
-    for (int d = 0; d < 8; d++) {
-        int cx = icol + kDX[d], cy = irow + kDY[d];
-        if (cx < 0 || cx >= W || cy < 0 || cy >= H) continue;
-        long nb = (long)cy * W + cx;
-        // a neighbour outside the cell's own DEM is off-grid for that DEM
-        if (demid && demid[nb] != demid[c]) continue;
-        int other = ws[nb];
-        if (other == me) continue;
-        if (!(zc >= z[nb])) continue;
-        bool dup = false;
-        for (int q = 0; q < ns; q++) dup |= (seen[q] == other);
-        if (dup) continue;
-        seen[ns++] = other;
-        unsigned lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
-        unsigned long long key = ((unsigned long long)lo << 32) | hi;
+    if (icol < W && irow < H && mask[c]) {
+        int me = ws[c];
+        double zc = z[c];
         unsigned long long wenc = ord_enc(zc);
-        unsigned long long m = (unsigned long long)(cap - 1);
-        unsigned long long h = mix64(key) & m;
-        long probes = 0;
-        while (true) {
-            unsigned long long prev = atomicCAS(&keys[h], HASH_EMPTY, key);
-            if (prev == HASH_EMPTY || prev == key) { atomicMin(&vals[h], wenc); break; }
-            h = (h + 1) & m;
-            if (++probes >= cap) { *overflow = 1; break; }
+        int seen[8];
+        int ns = 0;
+        for (int d = 0; d < 8; d++) {
+            int cx = icol + kDX[d], cy = irow + kDY[d];
+            if (cx < 0 || cx >= W || cy < 0 || cy >= H) continue;
+            long nb = (long)cy * W + cx;
+            // a neighbour outside the cell's own DEM is off-grid for that DEM
+            if (demid && demid[nb] != demid[c]) continue;
+            int other = ws[nb];
+            if (other == me) continue;
+            if (!(zc >= z[nb])) continue;
+            bool dup = false;
+            for (int q = 0; q < ns; q++) dup |= (seen[q] == other);
+            if (dup) continue;
+            seen[ns++] = other;
+            unsigned lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
+            unsigned long long key = ((unsigned long long)lo << 32) | hi;
+            unsigned h = (unsigned)(mix64(key) & (EDGE_LDS - 1));
+            bool done = false;
+            for (int p = 0; p < 16 && !done; p++) {
+                unsigned long long prev = atomicCAS(&lk[h], HASH_EMPTY, key);
+                if (prev == HASH_EMPTY || prev == key) { atomicMin(&lv[h], wenc); done = true; }
+                h = (h + 1) & (EDGE_LDS - 1);
+            }
+            if (!done) edge_insert_global(key, wenc, keys, vals, cap, overflow);
         }
+    }
+    __syncthreads();
+    for (int q = tid; q < EDGE_LDS; q += 256) {
+        unsigned long long key = lk[q];
+        if (key != HASH_EMPTY) edge_insert_global(key, lv[q], keys, vals, cap, overflow);
     }
 }
 
@@ -780,7 +843,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     if (n == 0) return LEMHIP_OK;
     prof_begin(c, PROF_PLATEAU);
     const int ntx = (W + PTW - 1) / PTW, nty = (H + PTH - 1) / PTH, ntiles = ntx * nty;
-    int *tile_has = c->pl_tiles, *tstamp = c->pl_tiles + ntiles;
+    int *tile_has = c->pl_tiles, *tstamp = c->pl_tiles + ntiles, *tver = c->pl_tiles + 5 * (long)ntiles;
     int *L[3] = {c->pl_tiles + 2 * (long)ntiles, c->pl_tiles + 3 * (long)ntiles, c->pl_tiles + 4 * (long)ntiles};
     PlatCtl *ctl = reinterpret_cast<PlatCtl *>(c->pl_ctl);
     float *sp_in = c->sp0, *sp_out = c->sp1;
@@ -792,6 +855,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     LEM_CHECK(c, hipMemsetAsync(tile_has, 0, ntiles * sizeof(int), c->s));
     LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(PlatCtl), c->s));
     k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, sp_in);
+    k_plat_tver_init<<<(ntiles + 255) / 256, 256, 0, c->s>>>(ntiles, tver);
     k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, c->cand, n, sp_in, ntx, tile_has);
     k_plat_first_list<<<grid1d(ntiles, 256, 1024), 256, 0, c->s>>>(ntiles, tile_has, L[launch % 3], ctl, launch % 3);
     LEM_LAUNCHED(c);
@@ -804,15 +868,14 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
             const int *a_cur = L[launch % 3];
             int *a_next = L[(launch + 1) % 3];
             switch (c->plat_k) {
-#define PLAT_LAUNCH(K)                                                                                               \
-    k_plat_block<K, TS><<<G, 256, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, eqb, c->fd, c->SFD, slopes, \
-                                             c->NP, tile_has, tstamp, ctl)
+#define PLAT_LAUNCH(K)                                                                                           \
+    k_plat_block<K, TS><<<G, 256, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, tver, eqb, c->fd, c->SFD, \
+                                             slopes, c->NP, tile_has, tstamp, ctl)
             case 4: PLAT_LAUNCH(4); break;
             case 16: PLAT_LAUNCH(16); break;
             default: PLAT_LAUNCH(8); break;
 #undef PLAT_LAUNCH
             }
-            k_plat_publish<<<G, 256, 0, c->s>>>(W, H, launch, a_cur, sp_out, sp_in, ctl);
             ++launch;
             r += c->plat_k;
         }
