@@ -5,7 +5,7 @@ Workloads (all with state resident in HBM, 30 m cells, SFD routing, default
 parameters, soil thickness from the reference formula clamped at 0 — the
 unclamped formula turns every cell below 400 m into NaN after one step):
 
-* value — batched FP32 + compensated accumulation: B (default 4) independent
+* value — batched FP32 + compensated accumulation: B (default 8) independent
   4096 x 4096 fBm DEMs (diamond-square H = 0.8, 0-800 m, 0.5 % tilt, 400 pits,
   100 plateaus, 5 % nodata frame, per-cell lognormal(0, 0.3) rain rescaled to
   a 1000 mm/yr mean; seeds differ per DEM) packed in one atlas grid and
@@ -164,15 +164,16 @@ def roofline(prof, steps, cells, t, tuning):
             "ms_per_step": ms / steps}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batched_4x4096_fp32_traffic.json")
+def traffic_file(batch):
+    return os.path.join(ROOT, "profiles", f"batched_{batch}x4096_fp32_traffic.json")
 
 
-def measured_traffic(kernel, tuning):
+def measured_traffic(kernel, tuning, batch):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
     WRITE_SIZE counter runs of the same workload and tuning (calibrated
     against a copy kernel of known size), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(batch)) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -201,7 +202,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=4, help="4096^2 DEMs in the batched form")
+    ap.add_argument("--batch", type=int, default=8, help="4096^2 DEMs in the batched form")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-DEM forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-side", type=int, default=2048)
@@ -265,7 +266,7 @@ def main():
                          f"(C restatement, OpenMP), {dt:.1f} s", "host": platform.processor() or platform.machine()}
 
     roof = roofline(profb, args.steps, atlas_dims[0] * atlas_dims[1], 4, summb[0]["tuning"])
-    roof["traffic"] = measured_traffic(roof["kernel"], summb[0]["tuning"])
+    roof["traffic"] = measured_traffic(roof["kernel"], summb[0]["tuning"], args.batch)
     if roof["traffic"] is not None:
         roof["traffic_note"] = "HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"
     out = {
