@@ -237,6 +237,11 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     c->NP = c->N;
     c->tsz = precision == LEMHIP_FP64 ? 8 : 4;
     if (hipSetDevice(device_id) != hipSuccess) { c->err = "hipSetDevice failed"; return LEMHIP_E_HIP; }
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device_id) == hipSuccess && ncu > 0)
+            c->ncu = ncu;
+    }
     LEM_CHECK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     long N = c->N;
     size_t T = c->tsz;

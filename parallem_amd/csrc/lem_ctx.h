@@ -47,6 +47,7 @@ struct lemhip_ctx {
     lem_params p;
     int prec = LEMHIP_FP64;
     int dev = 0;
+    int ncu = 256;        // compute units of the device
     hipStream_t s = nullptr;
     int W = 0, H = 0;     // grid rows/cells a step covers (an atlas steps only its used rows)
     long N = 0;

@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
     __shared__ uint8_t lfd[NC];
     __shared__ uint8_t hst[NH];
     __shared__ typename Cell::Lds lds;
-    __shared__ int spill[2][NC];
+    __shared__ uint16_t spill[2][NC];
     __shared__ int nspill[2];
     __shared__ int any_pending;
     __shared__ unsigned exits;
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
                             int rl = nr * TW + nc;
                             if (atomicSub(&lcnt[rl], 1) == 1) {
                                 if (nxt < 0) nxt = rl;
-                                else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = rl;
+                                else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = (uint16_t)rl;
                             }
                         } else {
                             // receiver in a neighbour tile: 3x3 position of that tile
@@ -570,7 +570,12 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
     c->hint[16] = ntiles;
     LEM_CHECK(c, hipMemcpyAsync(&ctl->cnt[(r + 1) % 3], &c->hint[16], sizeof(int), hipMemcpyHostToDevice, c->s));
     const int BATCH = c->dag_batch;
-    const int G = ntiles < 1024 ? ntiles : 1024;
+    // as many workgroups as the CUs hold at once (the first rounds visit every tile)
+    int bpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_dag_round<TW, TH, Cell>, 256, 0) != hipSuccess || bpc < 1)
+        bpc = 4;
+    int G = bpc * c->ncu;
+    if (G > ntiles) G = ntiles;
     long guard = 0;
     for (;;) {
         LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));

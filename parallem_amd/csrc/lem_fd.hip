@@ -47,6 +47,37 @@ __device__ __forceinline__ int block_append(int *counter, bool pred, int *s_wave
     return pred ? pos : -1;
 }
 
+// n_mine items per thread appended to a global list with one atomic per block;
+// returns this thread's first index
+__device__ __forceinline__ int block_append_n(int *counter, int n_mine, int *s_wave, int *s_base)
+{
+    int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    int lane = tid & 63;
+    int wave = tid >> 6;
+    int nwaves = (blockDim.x * blockDim.y) >> 6;
+    int x = n_mine;   // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    if (tid == 0) {
+        int tot = 0;
+        for (int w = 0; w < nwaves; w++) {
+            int cnt = s_wave[w];
+            s_wave[w] = tot;
+            tot += cnt;
+        }
+        *s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    int pos = *s_base + s_wave[wave] + x - n_mine;
+    __syncthreads();
+    return pos;
+}
+
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v, T *sh)
 {
@@ -137,6 +168,7 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
     }
     __syncthreads();
     const int icol = x0 + threadIdx.x;
+    unsigned cand_rows = 0;   // bit rg: the cell of row group rg is a plateau candidate
 #pragma unroll 1
     for (int rg = 0; rg < 4; rg++) {
         int lr = rg * 4 + threadIdx.y;
@@ -170,9 +202,14 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
             if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
         }
         if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; eqb[self] = (uint8_t)eq; }
-        int slot = block_append(ncand, active && aspect == 0 && aspectsfd == 0, s_wave, &s_base);
-        if (slot >= 0) cand[slot] = (int)self;
+        if (active && aspect == 0 && aspectsfd == 0) cand_rows |= 1u << rg;
     }
+    // one append per block for its candidates (row-major order within the block is not needed)
+    int mine = __popc(cand_rows);
+    int base = block_append_n(ncand, mine, s_wave, &s_base);
+#pragma unroll
+    for (int rg = 0; rg < 4; rg++)
+        if ((cand_rows >> rg) & 1) cand[base++] = (int)((long)(y0 + rg * 4 + threadIdx.y) * W + icol);
 }
 
 // ------------------------------------------------------ A3/A4: plateaus
@@ -473,6 +510,49 @@ __global__ void k_sink_merge(int W, int H, const int *cand, int n, const uint8_t
         }
     }
     if (best > cur) { ws[self] = best; *flag = 1; }
+}
+
+// Only sinks with an equal-height neighbour can change in the merge above;
+// they are collected once (the equal-height bits come from the FlowDirs pass on
+// the same surface) and the merge iterates over them alone.
+__global__ void __launch_bounds__(256) k_merge_list(const int *cand, int n, const uint8_t *SFD, const uint8_t *eqb,
+                                                    int *list, int *count)
+{
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {
+        int k = k0 + threadIdx.x;
+        int i = k < n ? cand[k] : 0;
+        bool keep = k < n && SFD[i] == 0 && eqb[i] != 0;
+        int at = block_append(count, keep, s_wave, &s_base);
+        if (keep) list[at] = i;
+    }
+}
+
+// cells still pointing at another cell after the first jumps
+__global__ void __launch_bounds__(256) k_jump_list(long N, const uint8_t *mask, const int *ws, int *list, int *count)
+{
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    for (long i0 = blockIdx.x * (long)blockDim.x; i0 < N; i0 += (long)gridDim.x * blockDim.x) {
+        long i = i0 + threadIdx.x;
+        bool keep = i < N && mask[i] && ws[i] < 0;
+        int at = block_append(count, keep, s_wave, &s_base);
+        if (keep) list[at] = (int)i;
+    }
+}
+
+__global__ void k_jump_some(int n, const int *list, int *ws, int *flag)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int i = list[k];
+        int v = ws[i];
+        if (v < 0) {
+            int t = ws[-v - 1];
+            ws[i] = t;
+            if (t < 0) *flag = 1;
+        }
+    }
 }
 
 // identify_watershedMFD (newflow.cu:469-487): pointer jumping; in-place reads
@@ -859,7 +939,16 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, c->cand, n, sp_in, ntx, tile_has);
     k_plat_first_list<<<grid1d(ntiles, 256, 1024), 256, 0, c->s>>>(ntiles, tile_has, L[launch % 3], ctl, launch % 3);
     LEM_LAUNCHED(c);
-    const int G = ntiles < 1024 ? ntiles : 1024;
+    int bpc = 0;
+    hipError_t oe = hipSuccess;
+    switch (c->plat_k) {
+    case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<4, TS>, 256, 0); break;
+    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<16, TS>, 256, 0); break;
+    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<8, TS>, 256, 0); break;
+    }
+    if (oe != hipSuccess || bpc < 1) bpc = 4;
+    int G = bpc * c->ncu;
+    if (G > ntiles) G = ntiles;
     const int BATCH = 8;
     int r = 0;  // rounds done in this pass
     long guard = 0;
@@ -999,17 +1088,29 @@ static int flooding(lemhip_ctx *c, int ncand1)
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     int nlab = c->hint[0] + c->hint[1];
     c->h_nlab = nlab;
-    for (int it = 0;; it += 4) {
-        LEM_TRY(clear_flag(c));
-        for (int q = 0; q < 4; q++)
-            k_sink_merge<<<grid1d(ncand1, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, ncand1, c->SFD, c->dem,
-                                                                         c->ws, &c->scal->flag);
+    // sink merge over the sinks that have an equal-height neighbour (list in the
+    // prefix buffer, free after k_init_ws)
+    {
+        int *mlist = reinterpret_cast<int *>(c->z2);
+        LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
+        k_merge_list<<<grid1d(ncand1, 256, 4096), 256, 0, c->s>>>(c->cand, ncand1, c->SFD, c->dmask, mlist,
+                                                                 &c->scal->ncand);
         LEM_LAUNCHED(c);
-        int f = 0;
-        LEM_TRY(read_flag(c, &f));
-        if (!f) break;
-        if (it / 4 > ncand1 + 2) { c->err = "sink merge did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        LEM_TRY(read_scalars(c));
+        int nm = c->hscal->ncand;
+        for (int it = 0; nm > 0; it += 4) {
+            LEM_TRY(clear_flag(c));
+            for (int q = 0; q < 4; q++)
+                k_sink_merge<<<grid1d(nm, 256, 1 << 30), 256, 0, c->s>>>(W, H, mlist, nm, c->SFD, c->dem, c->ws,
+                                                                         &c->scal->flag);
+            LEM_LAUNCHED(c);
+            int f = 0;
+            LEM_TRY(read_flag(c, &f));
+            if (!f) break;
+            if (it / 4 > nm + 2) { c->err = "sink merge did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        }
     }
+    // pointer jumping: four passes over every cell, then only the cells still pointing
     for (int it = 0;; it += 4) {
         LEM_TRY(clear_flag(c));
         for (int q = 0; q < 4; q++) k_jump<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, &c->scal->flag);
@@ -1017,11 +1118,26 @@ static int flooding(lemhip_ctx *c, int ncand1)
         int f = 0;
         LEM_TRY(read_flag(c, &f));
         if (!f) break;
-        if (it > 128) { c->err = "watershed pointer jumping did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        int *jlist = c->scan_tmp;
+        LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
+        k_jump_list<<<grid1d(N, 256, 8192), 256, 0, c->s>>>(N, c->mask, c->ws, jlist, &c->scal->ncand);
+        LEM_LAUNCHED(c);
+        LEM_TRY(read_scalars(c));
+        int nj = c->hscal->ncand;
+        for (int it2 = 0; nj > 0; it2 += 4) {
+            LEM_TRY(clear_flag(c));
+            for (int q = 0; q < 4; q++)
+                k_jump_some<<<grid1d(nj, 256, 8192), 256, 0, c->s>>>(nj, jlist, c->ws, &c->scal->flag);
+            LEM_LAUNCHED(c);
+            LEM_TRY(read_flag(c, &f));
+            if (!f) break;
+            if (it2 > 128) { c->err = "watershed pointer jumping did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        }
+        break;
     }
     // edges
     LEM_TRY(ensure_labels(c, (long)nlab + 1));
-    long want = 8L * (nlab + 1);
+    long want = 4L * (nlab + 1);   // about three edges per basin
     if (want > 4 * N) want = 4 * N;
     if (want < 1024) want = 1024;
     for (int attempt = 0;; ++attempt) {
