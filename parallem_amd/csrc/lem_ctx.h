@@ -26,7 +26,7 @@ struct ProfSlot {
 #define PTH 32
 
 // per-cell status of the dependency engine
-enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2 };
+enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2, ST_RING_WAIT = 3 };
 
 // Shelf packing of DEM rectangles into the atlas grid.  Each request is
 // padded by one nodata column on the right and one nodata row below, so no

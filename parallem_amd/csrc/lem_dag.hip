@@ -33,23 +33,43 @@ __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
     return make_float2(hi, lo);
 }
 
+// Packed LDS word of a tile cell during a round:
+//   bits  0-7   donors in the tile not yet resolved; W_NEVER for a cell that
+//               no decrement may hand over (not pending, parked, or dead
+//               from the start): at most 8 decrements never reach zero
+//   bits  8-15  donor bits (pull order)      bits 16-23  receiver bits (fd)
+//   bits 24-27  dead donors seen             bits 28-29  state
+//   bit  30     the sediment stone update applies
+// A resolved cell hands its receivers one atomic add each (-1, or -1 plus one
+// dead donor), and the add that brings a counter to zero returns the whole
+// word of the receiver, so the next link of the chain starts with its donor
+// and receiver bits and its dead flag already in registers.
+#define W_CNT 0xFFu
+#define W_NEVER 0x80u
+#define W_DEAD1 (1u << 24)
+#define W_STSH 28
+#define W_UPD (1u << 30)
+
 // ------------------------------------------------------------ cell bodies
 //
 // A cell body caches in LDS, per tile cell, what its receivers in the same
 // tile will read, plus the same values for the one-cell ring around the tile
 // (donors in other tiles, completed in an earlier launch), and evaluates one
 // cell once all donors are final.  Walks then never wait on global loads.
+// Values that receivers read are kept in a padded (TW+2) x (TH+2) layout that
+// holds the ring too, so a donor is one LDS read at a fixed offset from the
+// cell's padded index; values only the cell itself reads are unpadded.
 
-// ring index of a neighbour (nr, nc) just outside a TW x TH tile: top row,
-// bottom row (both with corners), left column, right column
-template <int TW, int TH>
-__device__ __forceinline__ int halo_idx(int nr, int nc)
-{
-    if (nr < 0) return nc + 1;
-    if (nr >= TH) return (TW + 2) + nc + 1;
-    if (nc < 0) return 2 * (TW + 2) + nr;
-    return 2 * (TW + 2) + TH + nr;
-}
+__host__ __device__ constexpr int cdx(int d) { return (d >= 1 && d <= 3) ? 1 : ((d >= 5) ? -1 : 0); }
+__host__ __device__ constexpr int cdy(int d) { return (d == 7 || d <= 1) ? -1 : ((d >= 3 && d <= 5) ? 1 : 0); }
+// pull order of the donor bits: E, SE, S, SW, W, NW, N, NE (kPULL)
+__host__ __device__ constexpr int cpull(int k) { return (k + 2) & 7; }
+// padded index of neighbour d of the cell at padded index p
+template <int TW> __host__ __device__ constexpr int poff(int d) { return cdy(d) * (TW + 2) + cdx(d); }
+template <int TW> __device__ __forceinline__ int pidx(int lr, int lc) { return (lr + 1) * (TW + 2) + lc + 1; }
+
+// position (nr, nc) relative to the tile of ring cell h of a TW x TH tile: top
+// row, bottom row (both with corners), left column, right column
 template <int TW, int TH>
 __device__ __forceinline__ void halo_pos(int h, int *nr, int *nc)
 {
@@ -66,18 +86,27 @@ __device__ __forceinline__ void fa_clamp(float2 &a) { if (a.x < 0.0f) a = make_f
 __device__ __forceinline__ double fa_final(double w, double a) { double v = w; v += a; return v; }
 __device__ __forceinline__ float2 fa_final(float w, float2 a) { return ff_add(make_float2(w, 0.f), a); }
 
+// Cell body interface used by the round kernel (li: index in the tile, p:
+// padded index):
+//   Reg  load(i)        the cell's own inputs (tile cells, loaded in phase 1)
+//   HReg load_halo(i)   what a receiver reads of a donor in another tile
+//   Fin  store(L, li, p, r, &extra)  put the walk-visible values into LDS and
+//        return what the cell writes to HBM when it completes that does not
+//        depend on its donors (kept in registers); extra: word flags
+//   compute(L, li, p, bits, upd)  the chain part, LDS only
+//   finish(L, li, p, i, fin, upd)  HBM writes of a cell completed in this
+//        round, after the walks (coalesced, off the chains)
 template <typename T, int TW, int TH>
 struct FaCell {
     using FS = typename FaStore<T>::type;
     static constexpr int NC = TW * TH;
-    static constexpr int NH = 2 * (TW + 2) + 2 * TH;
+    static constexpr int NP = (TW + 2) * (TH + 2);
     static constexpr bool kCanDie = false;
     const T *w;
     FS *fa;
     struct Lds {
-        FS val[NC];
+        FS val[NP];
         T w[NC];
-        FS hval[NH];
     };
     // loads are issued unconditionally and stored to LDS after one wait; the
     // value of a cell that is not final yet is never read
@@ -85,122 +114,110 @@ struct FaCell {
         T w;
         FS v;
     };
+    using HReg = FS;
+    struct Fin {};
     __device__ __forceinline__ Reg load(long i) const { return Reg{w[i], fa[i]}; }
-    __device__ __forceinline__ void store(Lds &L, int li, const Reg &r) const
+    __device__ __forceinline__ HReg load_halo(long i) const { return fa[i]; }
+    __device__ __forceinline__ Fin store(Lds &L, int li, int p, const Reg &r, uint32_t *) const
     {
         L.w[li] = r.w;
-        L.val[li] = r.v;
+        L.val[p] = r.v;
+        return Fin{};
     }
-    __device__ __forceinline__ Reg load_halo(long i) const { return Reg{T(0), fa[i]}; }
-    __device__ __forceinline__ void store_halo(Lds &L, int h, const Reg &r) const { L.hval[h] = r.v; }
+    __device__ __forceinline__ void store_halo(Lds &L, int p, const HReg &r) const { L.val[p] = r; }
     // fa = w + max(0, sum of donor fa in the order E..NE) (mfd_simple.cu:71-168;
     // prop = 1 for every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
-    __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const
+    __device__ __forceinline__ void compute(Lds &L, int li, int p, unsigned bits, bool) const
     {
-        (void)W;
         FS accum{};
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            if (bits & (1u << k)) {
-                int d = kPULL[k];
-                int nr = lr + kDY[d], nc = lc + kDX[d];
-                FS v = (nr >= 0 && nr < TH && nc >= 0 && nc < TW) ? L.val[nr * TW + nc] : L.hval[halo_idx<TW, TH>(nr, nc)];
-                fa_acc(accum, v);
-            }
-        }
+        for (int k = 0; k < 8; k++)
+            if (bits & (1u << k)) fa_acc(accum, L.val[p + poff<TW>(cpull(k))]);
         fa_clamp(accum);
-        FS v = fa_final(L.w[li], accum);
-        fa[i] = v;
-        L.val[li] = v;
+        L.val[p] = fa_final(L.w[li], accum);
     }
+    __device__ __forceinline__ void finish(const Lds &L, int, int p, long i, const Fin &, bool) const { fa[i] = L.val[p]; }
 };
 
+// MFDsedbud body (mfd_accum.cu:63-252) split by what it reads:
+//  * deposition from the donors' erosion, incision and gelifluction, which
+//    are inputs of this pass: computed for every cell by the prep kernel
+//    into a scratch array (dps);
+//  * the new soil thickness and the weights of the stone update, from the
+//    cell's own values: computed when the tile is loaded;
+//  * the stone update, the only term that reads donors' results:
+//      stone = a + b * (sum of the donors' updated stone, order E..NE)
+//    with a = old * stone and b = 1 - old (old = soilT / sediment thickness),
+//    the same operations and roundings as the body written in one piece.
 template <typename T, int TW, int TH>
 struct SedCell {
     static constexpr int NC = TW * TH;
-    static constexpr int NH = 2 * (TW + 2) + 2 * TH;
+    static constexpr int NP = (TW + 2) * (TH + 2);
     static constexpr bool kCanDie = true;
-    const T *ero, *inci, *geli;
+    const T *ero, *inci, *geli, *dps;
     T *depo, *stone, *fines;
     double *soilT;
-    double ddk, dck, dgk;
     struct Lds {
-        double sT[NC];
-        T ero[NC], inci[NC], geli[NC], stone[NC];
-        T hero[NH], hinci[NH], hgeli[NH], hstone[NH];
+        T stone[NP];
+        T a[NC], b[NC];
     };
     struct Reg {
-        T e, in, g, s;
+        T e, in, g, s, dp;
         double sT;
     };
-    __device__ __forceinline__ Reg load(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i], soilT[i]}; }
-    __device__ __forceinline__ void store(Lds &L, int li, const Reg &r) const
+    using HReg = T;
+    struct Fin {
+        T dp;
+        double sTn;
+    };
+    __device__ __forceinline__ Reg load(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i], dps[i], soilT[i]}; }
+    __device__ __forceinline__ HReg load_halo(long i) const { return stone[i]; }
+    __device__ __forceinline__ Fin store(Lds &L, int li, int p, const Reg &r, uint32_t *extra) const
     {
-        L.ero[li] = r.e;
-        L.inci[li] = r.in;
-        L.geli[li] = r.g;
-        L.stone[li] = r.s;
-        L.sT[li] = r.sT;
-    }
-    __device__ __forceinline__ Reg load_halo(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i], 0.0}; }
-    __device__ __forceinline__ void store_halo(Lds &L, int h, const Reg &r) const
-    {
-        L.hero[h] = r.e;
-        L.hinci[h] = r.in;
-        L.hgeli[h] = r.g;
-        L.hstone[h] = r.s;
-    }
-    // MFDsedbud body (mfd_accum.cu:63-252); donors' stone is their updated value
-    __device__ __forceinline__ void compute(Lds &L, int li, long i, int W, unsigned bits, int lr, int lc) const
-    {
-        (void)W;
-        T accum_d = 0, accum_c = 0, accum_g = 0;
-        T from[8];
-        const T one = (T)1;
-        const T kd = (T)(1 - ddk), kc = (T)(1 - dck), kg = (T)(1 - dgk);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            from[k] = 0;
-            if (bits & (1u << k)) {
-                int d = kPULL[k];
-                int nr = lr + kDY[d], nc = lc + kDX[d];
-                T e, in, g, sn;
-                if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
-                    int nl = nr * TW + nc;
-                    e = L.ero[nl]; in = L.inci[nl]; g = L.geli[nl]; sn = L.stone[nl];
-                } else {
-                    int h = halo_idx<TW, TH>(nr, nc);
-                    e = L.hero[h]; in = L.hinci[h]; g = L.hgeli[h]; sn = L.hstone[h];
-                }
-                accum_d += one * (e * kd);
-                accum_c += one * (in * kc);
-                accum_g += one * (g * kg);
-                from[k] = one * sn;
-            }
-        }
-        T er = L.ero[li], ic = L.inci[li], ge = L.geli[li];
-        T dp = ((accum_d + er) * (T)ddk) + ((accum_c + ic) * (T)dck) + ((accum_g + ge) * (T)dgk);
-        depo[i] = dp;
-        double sT = L.sT[li];
-        double sedthick = sT + (double)dp;
+        L.stone[p] = r.s;
+        double sT = r.sT;
+        double sedthick = sT + (double)r.dp;
+        T a = 0, b = 0;
         if (sedthick != 0.0) {
             double old = sT / sedthick;
             double input = 1 - old;
-            T propStonefrom = (from[0] + from[1] + from[2] + from[3] + from[4] + from[5] + from[6] + from[7]);
-            T st;
-            if (sizeof(T) == 8) st = (T)((old * (double)L.stone[li]) + (input * (double)propStonefrom));
-            else st = (T)(((T)old * L.stone[li]) + ((T)input * propStonefrom));
-            if (st < (T)0.01) st = (T)0.01;
-            else if (st > (T)99.9) st = (T)99.9;
+            if (sizeof(T) == 8) { a = (T)(old * (double)r.s); b = (T)input; }
+            else { a = (T)old * r.s; b = (T)input; }
+            *extra |= W_UPD;
+        }
+        L.a[li] = a;
+        L.b[li] = b;
+        double toterosion;
+        if (sizeof(T) == 8) toterosion = (double)r.e + (double)r.in + (double)r.g;
+        else toterosion = (double)(r.e + r.in + r.g);
+        Fin f{r.dp, sT};
+        if (toterosion >= sedthick) f.sTn = 0.0;
+        if (sedthick > toterosion) f.sTn = sedthick - toterosion;
+        return f;
+    }
+    __device__ __forceinline__ void store_halo(Lds &L, int p, const HReg &r) const { L.stone[p] = r; }
+    __device__ __forceinline__ void compute(Lds &L, int li, int p, unsigned bits, bool upd) const
+    {
+        if (!upd) return;
+        const T one = (T)1;
+        T from[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) from[k] = (bits & (1u << k)) ? one * L.stone[p + poff<TW>(cpull(k))] : (T)0;
+        T propStonefrom = (from[0] + from[1] + from[2] + from[3] + from[4] + from[5] + from[6] + from[7]);
+        T st = L.a[li] + L.b[li] * propStonefrom;
+        if (st < (T)0.01) st = (T)0.01;
+        else if (st > (T)99.9) st = (T)99.9;
+        L.stone[p] = st;
+    }
+    __device__ __forceinline__ void finish(const Lds &L, int, int p, long i, const Fin &f, bool upd) const
+    {
+        depo[i] = f.dp;
+        if (upd) {
+            T st = L.stone[p];
             stone[i] = st;
-            L.stone[li] = st;
             fines[i] = (T)100 - st;
         }
-        double toterosion;
-        if (sizeof(T) == 8) toterosion = (double)er + (double)ic + (double)ge;
-        else toterosion = (double)(er + ic + ge);
-        if (toterosion >= sedthick) soilT[i] = 0.0;
-        if (sedthick > toterosion) soilT[i] = sedthick - toterosion;
+        soilT[i] = f.sTn;
     }
 };
 
@@ -233,10 +250,16 @@ __global__ void k_fa_prep(int W, int H, const uint8_t *__restrict__ mask, const 
 // MFDsedbud never computes row 0 / column 0 (mfd_accum.cu:42) and nodata
 // cells are never ok (mfd_accum.cu:390-394); nodata carries fd = -9999, whose
 // low byte sets E, W, NW, N, NE, so a nodata (or, D11, off-grid) neighbour at
-// E, SE, S, SW or W is a donor that never completes (D9)
+// E, SE, S, SW or W is a donor that never completes (D9).  Also the
+// deposition of every cell (mfd_accum.cu:63-200): the donors' erosion,
+// incision and gelifluction terms in the order E..NE plus the cell's own, each
+// weighted by its deposition fraction; it reads only inputs of the pass.
+template <typename T>
 __global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ fd,
                            const uint8_t *__restrict__ demid, const DemDesc *__restrict__ dd,
-                           uint8_t *__restrict__ dm, int *__restrict__ st)
+                           uint8_t *__restrict__ dm, int *__restrict__ st, const T *__restrict__ ero,
+                           const T *__restrict__ inci, const T *__restrict__ geli, double ddk, double dck, double dgk,
+                           T *__restrict__ dps)
 {
     int icol = blockIdx.x * blockDim.x + threadIdx.x;
     int irow = blockIdx.y * blockDim.y + threadIdx.y;
@@ -261,6 +284,19 @@ __global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const
             if (fd[n] & donor_code(d)) bits |= 1u << k;
         }
         s = dead ? -1 : DAG_PENDING;
+        T accum_d = 0, accum_c = 0, accum_g = 0;
+        const T one = (T)1;
+        const T kd = (T)(1 - ddk), kc = (T)(1 - dck), kg = (T)(1 - dgk);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (!(bits & (1u << k))) continue;
+            int d = kPULL[k];
+            long n = (long)(irow + kDY[d]) * W + (icol + kDX[d]);
+            accum_d += one * (ero[n] * kd);
+            accum_c += one * (inci[n] * kc);
+            accum_g += one * (geli[n] * kg);
+        }
+        dps[i] = ((accum_d + ero[i]) * (T)ddk) + ((accum_c + inci[i]) * (T)dck) + ((accum_g + geli[i]) * (T)dgk);
     }
     dm[i] = (uint8_t)bits;
     st[i] = s;
@@ -279,7 +315,6 @@ struct DagCtl {
     int pad1[31];
 };
 
-#define DAG_BIG (1 << 24)
 
 // Inside a tile, donors are read from LDS only (values, states, counters);
 // HBM copies are read by other tiles in later launches.  So ordering a
@@ -296,7 +331,9 @@ __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)
 // ready receivers (MFD) go to an LDS spill list processed after a barrier.
 // A chain leaving the tile only marks the neighbour tile in an LDS mask; the
 // tile wakes each marked neighbour once, after its walks.  A cell with a dead
-// donor dies (sediment blocking, D9).
+// donor dies (sediment blocking, D9).  Work of a cell that does not depend on
+// its donors' results (the cell body's pre-phase) is done for all cells of
+// the tile in parallel before the walks.
 template <int TW, int TH, class Cell>
 __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
                                                    const uint8_t *__restrict__ fdv, int *__restrict__ st,
@@ -305,14 +342,14 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
 {
     constexpr int NC = TW * TH;
     constexpr int PER = NC / 256;
-    constexpr int NH = Cell::NH;
+    constexpr int NH = 2 * (TW + 2) + 2 * TH;
     constexpr int HPER = (NH + 255) / 256;
     static_assert(NC % 256 == 0, "tile shape");
-    __shared__ int lcnt[NC];
-    __shared__ uint8_t lst[NC];
-    __shared__ uint8_t ldm[NC];
-    __shared__ uint8_t lfd[NC];
-    __shared__ uint8_t hst[NH];
+    __shared__ uint32_t wd[NC];
+    // states at the start of the round, padded layout: tile cells ST_PENDING,
+    // ST_DONE, ST_DEAD; ring cells the same, with a donor in another tile
+    // that is not final from an earlier launch as ST_RING_WAIT
+    __shared__ uint8_t pst[(TW + 2) * (TH + 2)];
     __shared__ typename Cell::Lds lds;
     __shared__ uint16_t spill[2][NC];
     __shared__ int nspill[2];
@@ -350,12 +387,12 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
             }
         }
         int hv[HPER];
-        typename Cell::Reg hr[HPER];
+        typename Cell::HReg hr[HPER];
 #pragma unroll
         for (int q = 0; q < HPER; q++) {
             int hh = threadIdx.x + q * 256;
             hv[q] = 0;   // off-grid ring cells are never donors
-            hr[q] = typename Cell::Reg{};
+            hr[q] = typename Cell::HReg{};
             if (hh < NH) {
                 int nr, nc;
                 halo_pos<TW, TH>(hh, &nr, &nc);
@@ -367,16 +404,18 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
                 }
             }
         }
+        typename Cell::Fin fin[PER];
+        uint32_t xw[PER];
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             int li = threadIdx.x + q * 256;
             int v = vs[q];
-            uint8_t sv = (v == DAG_PENDING) ? ST_PENDING : (v >= 0 ? ST_DONE : ST_DEAD);
-            cell.store(lds, li, vr[q]);
-            lst[li] = sv;
-            ldm[li] = vb[q];
-            lfd[li] = vf[q];
-            lcnt[li] = DAG_BIG;
+            uint32_t sv = (v == DAG_PENDING) ? ST_PENDING : (v >= 0 ? ST_DONE : ST_DEAD);
+            xw[q] = 0;
+            const int p = pidx<TW>(li / TW, li % TW);
+            fin[q] = cell.store(lds, li, p, vr[q], &xw[q]);
+            wd[li] = (sv << W_STSH) | ((uint32_t)vf[q] << 16) | ((uint32_t)vb[q] << 8) | W_NEVER;
+            pst[p] = (uint8_t)sv;
         }
 #pragma unroll
         for (int q = 0; q < HPER; q++) {
@@ -384,96 +423,94 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
             if (hh < NH) {
                 int v = hv[q];
                 uint8_t hs;
-                if (v == DAG_PENDING) hs = ST_PENDING;
-                else if (v >= 0) hs = (v < round) ? ST_DONE : ST_PENDING;
-                else hs = ((-v - 1) < round) ? ST_DEAD : ST_PENDING;
-                hst[hh] = hs;
-                cell.store_halo(lds, hh, hr[q]);
+                if (v == DAG_PENDING) hs = ST_RING_WAIT;
+                else if (v >= 0) hs = (v < round) ? ST_DONE : ST_RING_WAIT;
+                else hs = ((-v - 1) < round) ? ST_DEAD : ST_RING_WAIT;
+                int nr, nc;
+                halo_pos<TW, TH>(hh, &nr, &nc);
+                const int p = pidx<TW>(nr, nc);
+                pst[p] = hs;
+                cell.store_halo(lds, p, hr[q]);
             }
         }
         __syncthreads();
-        // 2. counters: donors unresolved in the tile; park cells waiting on other tiles
-        int seeds = 0;     // bit q: cell q of this thread starts a walk
-        int deadseeds = 0; // bit q: ... and it dies
+        // 2. counters: donors unresolved in the tile; park cells waiting on
+        //    other tiles
+        uint32_t myw[PER];
+        int seeds = 0;   // bit q: cell q of this thread starts a walk
+        int pend0 = 0;   // bit q: cell q was pending when the round started
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             int li = threadIdx.x + q * 256;
-            if (lst[li] != ST_PENDING) continue;
-            int lr = li / TW, lc = li % TW;
-            unsigned bits = ldm[li];
+            myw[q] = 0;
+            if (vs[q] != DAG_PENDING) continue;
+            pend0 |= 1 << q;
+            const int p = pidx<TW>(li / TW, li % TW);
+            unsigned bits = vb[q];
             int nin = 0;
             bool dead = false, parked = false;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 if (!(bits & (1u << k))) continue;
-                int d = kPULL[k];
-                int nr = lr + kDY[d], nc = lc + kDX[d];
-                bool inside = nr >= 0 && nr < TH && nc >= 0 && nc < TW;
-                uint8_t s2 = inside ? lst[nr * TW + nc] : hst[halo_idx<TW, TH>(nr, nc)];
-                if (s2 == ST_PENDING) { if (inside) nin++; else parked = true; }
-                else if (s2 == ST_DEAD) dead = true;
+                uint32_t s2 = pst[p + poff<TW>(cpull(k))];
+                nin += (s2 == ST_PENDING);
+                parked |= (s2 == ST_RING_WAIT);
+                dead |= (s2 == ST_DEAD);
             }
-            if (dead) { lcnt[li] = DAG_BIG; seeds |= 1 << q; deadseeds |= 1 << q; }
-            else if (!parked) { lcnt[li] = nin; if (nin == 0) seeds |= 1 << q; }
+            uint32_t base = ((uint32_t)ST_PENDING << W_STSH) | ((uint32_t)vf[q] << 16) | (bits << 8) | xw[q];
+            if (dead) { myw[q] = base | W_NEVER | W_DEAD1; seeds |= 1 << q; }
+            else if (!parked) { myw[q] = base | (uint32_t)nin; if (nin == 0) seeds |= 1 << q; }
+            else myw[q] = base | W_NEVER;
         }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; q++)
+            if ((pend0 >> q) & 1) wd[threadIdx.x + q * 256] = myw[q];
         __syncthreads();
         // 3. walks
         unsigned my_exits = 0;
-        int pass = 0;
-        for (;;) {
+        for (int pass = 0;; pass++) {
             int nwork = (pass == 0) ? 0 : nspill[(pass - 1) & 1];
             if (pass > 0 && nwork == 0) break;
             for (int w0 = 0;; w0++) {
                 int cur = -1;
-                bool seed_dead = false;
+                uint32_t word = 0;
                 if (pass == 0) {
                     if (w0 >= PER) break;
-                    if (!(seeds & (1 << w0))) continue;
+                    if (!((seeds >> w0) & 1)) continue;
                     cur = threadIdx.x + w0 * 256;
-                    seed_dead = (deadseeds >> w0) & 1;
+                    word = wd[cur];   // only dead-count and counter bits of a dead seed can have moved
                 } else {
                     int idx = threadIdx.x + w0 * 256;
                     if (idx >= nwork) break;
                     cur = spill[(pass - 1) & 1][idx];
+                    word = wd[cur];
                 }
                 while (cur >= 0) {
-                    int lr = cur / TW, lc = cur % TW;
-                    long i = (long)(ty0 + lr) * W + (tx0 + lc);
-                    unsigned bits = ldm[cur];
-                    lds_order();
-                    // a cell reached through its counter has every donor in other
-                    // tiles final (else it was parked); only donors in this tile can
-                    // have died since the counters were set
-                    bool dead = seed_dead;
-                    seed_dead = false;
-                    if (Cell::kCanDie && !dead) {
-#pragma unroll
-                        for (int k = 0; k < 8; k++) {
-                            if (!(bits & (1u << k))) continue;
-                            int d = kPULL[k];
-                            int nr = lr + kDY[d], nc = lc + kDX[d];
-                            if (nr >= 0 && nr < TH && nc >= 0 && nc < TW && lst[nr * TW + nc] == ST_DEAD) dead = true;
-                        }
+                    const int lr = cur / TW, lc = cur % TW;
+                    const bool dead = Cell::kCanDie && ((word >> 24) & 0xFu) != 0;
+                    uint32_t ns = ST_DEAD;
+                    if (!dead) {
+                        cell.compute(lds, cur, pidx<TW>(lr, lc), (word >> 8) & 0xFFu, (word & W_UPD) != 0);
+                        ns = ST_DONE;
                     }
-                    if (dead) {
-                        lst[cur] = ST_DEAD;
-                        st[i] = -(round + 1);
-                    } else {
-                        cell.compute(lds, cur, i, W, bits, lr, lc);
-                        lst[cur] = ST_DONE;
-                        st[i] = round;
-                    }
+                    wd[cur] = (word & ~(3u << W_STSH)) | (ns << W_STSH);
                     lds_order();
+                    // unsigned adds: -1, or -1 plus one dead donor (the counter
+                    // of a receiver is at least 1 here, so no borrow leaves it)
+                    const uint32_t delta = dead ? (W_DEAD1 - 1u) : 0xFFFFFFFFu;
+                    const unsigned f = (word >> 16) & 0xFFu;
                     int nxt = -1;
-                    unsigned f = lfd[cur];
+                    uint32_t nword = 0;
 #pragma unroll
                     for (int d = 0; d < 8; d++) {
-                        if (!(f & kCODE[d])) continue;
-                        int nr = lr + kDY[d], nc = lc + kDX[d];
+                        if (!(f & (1u << ((d + 6) & 7)))) continue;   // fd code of direction d
+                        int nr = lr + cdy(d), nc = lc + cdx(d);
                         if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
                             int rl = nr * TW + nc;
-                            if (atomicSub(&lcnt[rl], 1) == 1) {
-                                if (nxt < 0) nxt = rl;
+                            uint32_t old = atomicAdd(&wd[rl], delta);
+                            if ((old & W_CNT) == 1u) {
+                                if (nxt < 0) { nxt = rl; nword = old + delta; }
                                 else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = (uint16_t)rl;
                             }
                         } else {
@@ -483,18 +520,31 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
                         }
                     }
                     cur = nxt;
+                    word = nword;
                 }
             }
             __syncthreads();
             if (threadIdx.x == 0) nspill[(pass + 1) & 1] = 0;
             __syncthreads();
-            pass++;
         }
         if (my_exits) atomicOr(&exits, my_exits);
-        // 4. bookkeeping
+        // 4. HBM writes of the cells resolved in this round, and bookkeeping
         bool pend = false;
 #pragma unroll
-        for (int q = 0; q < PER; q++) pend |= (lst[threadIdx.x + q * 256] == ST_PENDING);
+        for (int q = 0; q < PER; q++) {
+            if (!((pend0 >> q) & 1)) continue;
+            int li = threadIdx.x + q * 256;
+            uint32_t wv = wd[li];
+            uint32_t s2 = (wv >> W_STSH) & 3u;
+            if (s2 == ST_PENDING) { pend = true; continue; }
+            long i = (long)(ty0 + li / TW) * W + (tx0 + li % TW);
+            if (s2 == ST_DONE) {
+                cell.finish(lds, li, pidx<TW>(li / TW, li % TW), i, fin[q], (wv & W_UPD) != 0);
+                st[i] = round;
+            } else {
+                st[i] = -(round + 1);
+            }
+        }
         if (pend) any_pending = 1;
         int prog = __syncthreads_or(seeds != 0);
         // wake each neighbour the chains flow into, once per round; wave 0
@@ -619,12 +669,10 @@ static int fa_shape(lemhip_ctx *c)
 template <typename T, int TW, int TH>
 static int sed_shape(lemhip_ctx *c)
 {
-    double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
-    if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407
-    if (dck > 0.9) dck = 0.9;
     SedCell<T, TW, TH> cell{reinterpret_cast<const T *>(c->ero), reinterpret_cast<const T *>(c->inci),
-                            reinterpret_cast<const T *>(c->geli), reinterpret_cast<T *>(c->depo),
-                            reinterpret_cast<T *>(c->stone), reinterpret_cast<T *>(c->fines), c->soilT, ddk, dck, dgk};
+                            reinterpret_cast<const T *>(c->geli), reinterpret_cast<const T *>(c->z2),
+                            reinterpret_cast<T *>(c->depo), reinterpret_cast<T *>(c->stone),
+                            reinterpret_cast<T *>(c->fines), c->soilT};
     return run_rounds<TW, TH>(c, cell, &c->h_sed_unc, PROF_SED_K);
 }
 
@@ -648,7 +696,14 @@ int lem_sed_dag(lemhip_ctx *c)
 {
     dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 3) / 4);
     int *st = reinterpret_cast<int *>(c->scan_tmp);
-    k_sed_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->demid, c->ddesc, c->dmask, st);
+    double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
+    if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407
+    if (dck > 0.9) dck = 0.9;
+    // deposition goes to the raised-surface buffer of the direction phase,
+    // free until the next step
+    k_sed_prep<T><<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->demid, c->ddesc, c->dmask, st,
+                                         (const T *)c->ero, (const T *)c->inci, (const T *)c->geli, ddk, dck, dgk,
+                                         (T *)c->z2);
     LEM_LAUNCHED(c);
     switch (c->dag_shape) {
 #define X(id, tw, th) case id: return sed_shape<T, tw, th>(c);
