@@ -60,10 +60,6 @@ __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
 // holds the ring too, so a donor is one LDS read at a fixed offset from the
 // cell's padded index; values only the cell itself reads are unpadded.
 
-__host__ __device__ constexpr int cdx(int d) { return (d >= 1 && d <= 3) ? 1 : ((d >= 5) ? -1 : 0); }
-__host__ __device__ constexpr int cdy(int d) { return (d == 7 || d <= 1) ? -1 : ((d >= 3 && d <= 5) ? 1 : 0); }
-// pull order of the donor bits: E, SE, S, SW, W, NW, N, NE (kPULL)
-__host__ __device__ constexpr int cpull(int k) { return (k + 2) & 7; }
 // padded index of neighbour d of the cell at padded index p
 template <int TW> __host__ __device__ constexpr int poff(int d) { return cdy(d) * (TW + 2) + cdx(d); }
 template <int TW> __device__ __forceinline__ int pidx(int lr, int lc) { return (lr + 1) * (TW + 2) + lc + 1; }

@@ -16,6 +16,10 @@ __device__ __constant__ static const int8_t kPULL[8] = {2, 3, 4, 5, 6, 7, 0, 1};
 
 // code a neighbour in direction d must carry to flow into the centre cell
 __device__ __forceinline__ uint8_t donor_code(int d) { return kCODE[(d + 4) & 7]; }
+// the same tables as compile-time functions, for fully unrolled loops
+__host__ __device__ constexpr int cdx(int d) { return (d >= 1 && d <= 3) ? 1 : ((d >= 5) ? -1 : 0); }
+__host__ __device__ constexpr int cdy(int d) { return (d == 7 || d <= 1) ? -1 : ((d >= 3 && d <= 5) ? 1 : 0); }
+__host__ __device__ constexpr int cpull(int k) { return (k + 2) & 7; }   // pull order E, SE, S, SW, W, NW, N, NE
 
 // direction index of an ESRI code (0 if not a single code)
 __device__ __forceinline__ int code_dir(int code)

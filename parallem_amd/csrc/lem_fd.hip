@@ -349,35 +349,42 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         }
         __syncthreads();
         float *cur = A, *nxt = B;
+        int moved = 0;   // last round in which one of this thread's tile cells changed
         for (int k = 1; k <= K; k++) {
 #pragma unroll
             for (int q = 0; q < PER; q++) {
-                if (!((own >> q) & 1)) continue;
-                int idx = threadIdx.x + q * 256;
-                int rr = idx / RW, cc = idx % RW;
-                if (rr < k || rr >= RH - k || cc < k || cc >= RW - k) continue;
-                float md = cur[idx];
-                unsigned ee = e[q];
-                int cd = 0;
-                unsigned mk = 0;
+                const int idx = threadIdx.x + q * 256;
+                const int rr = idx / RW, cc = idx % RW;
+                const bool act = ((own >> q) & 1) && rr >= k && rr < RH - k && cc >= k && cc < RW - k;
+                // the scan N..NW without branches; an inactive slot reads around
+                // a fixed interior cell and discards the result
+                const int b = act ? idx : RW + 1;
+                float md = cur[b];
+                const unsigned ee = E[b];   // read per round: keeps 8 lane masks per cell out of the registers
+                unsigned cd = 0, mk = 0;
 #pragma unroll
                 for (int d = 0; d < 8; d++) {
-                    if (!((ee >> d) & 1)) continue;
-                    float dc = (d & 1) ? 1.41f : 1.0f;
-                    float dis = cur[idx + kDY[d] * RW + kDX[d]] + dc;
-                    if (md > dis) { md = dis; cd = kCODE[d]; mk |= 1u << d; }
+                    const float dc = (d & 1) ? 1.41f : 1.0f;
+                    const float dis = cur[b + cdy(d) * RW + cdx(d)] + dc;
+                    const bool take = ((ee >> d) & 1u) && (md > dis);
+                    md = take ? dis : md;
+                    cd = take ? (1u << ((d + 6) & 7)) : cd;   // fd code of direction d
+                    mk |= take ? (1u << d) : 0u;
                 }
-                nxt[idx] = md;
-                if (cd && rr >= K && rr < K + PTH && cc >= K && cc < K + PTW) {
-                    int li = (rr - K) * PTW + (cc - K);
-                    code[li] = (uint8_t)cd;
-                    marks[li] |= (uint8_t)mk;
-                    atomicMax(&last, k);
+                if (act) {
+                    nxt[idx] = md;
+                    if (cd && rr >= K && rr < K + PTH && cc >= K && cc < K + PTW) {
+                        int li = (rr - K) * PTW + (cc - K);
+                        code[li] = (uint8_t)cd;
+                        marks[li] |= (uint8_t)mk;
+                        moved = k;
+                    }
                 }
             }
             __syncthreads();
             float *t = cur; cur = nxt; nxt = t;
         }
+        if (moved) atomicMax(&last, moved);
         // write back the tile: distances (into its other buffer), directions of
         // changed cells, slope marks (pass 1 only)
         float *sp_out = s_buf[4] ? sp0 : sp1;
