@@ -125,12 +125,18 @@ struct FaCell {
     // prop = 1 for every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
     __device__ __forceinline__ void compute(Lds &L, int li, int p, unsigned bits, bool) const
     {
+        // every neighbour and the weight in flight at once (the padded layout
+        // keeps all eight reads inside the array), then the donors in order
+        FS v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = L.val[p + poff<TW>(cpull(k))];
+        const T wv = L.w[li];
         FS accum{};
 #pragma unroll
         for (int k = 0; k < 8; k++)
-            if (bits & (1u << k)) fa_acc(accum, L.val[p + poff<TW>(cpull(k))]);
+            if (bits & (1u << k)) fa_acc(accum, v[k]);
         fa_clamp(accum);
-        L.val[p] = fa_final(L.w[li], accum);
+        L.val[p] = fa_final(wv, accum);
     }
     __device__ __forceinline__ void finish(const Lds &L, int, int p, long i, const Fin &, bool) const { fa[i] = L.val[p]; }
 };
@@ -196,11 +202,15 @@ struct SedCell {
     {
         if (!upd) return;
         const T one = (T)1;
+        T nb[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) nb[k] = L.stone[p + poff<TW>(cpull(k))];
+        const T a = L.a[li], b = L.b[li];
         T from[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) from[k] = (bits & (1u << k)) ? one * L.stone[p + poff<TW>(cpull(k))] : (T)0;
+        for (int k = 0; k < 8; k++) from[k] = (bits & (1u << k)) ? one * nb[k] : (T)0;
         T propStonefrom = (from[0] + from[1] + from[2] + from[3] + from[4] + from[5] + from[6] + from[7]);
-        T st = L.a[li] + L.b[li] * propStonefrom;
+        T st = a + b * propStonefrom;
         if (st < (T)0.01) st = (T)0.01;
         else if (st > (T)99.9) st = (T)99.9;
         L.stone[p] = st;
