@@ -505,12 +505,13 @@ __global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell
                     // unsigned adds: -1, or -1 plus one dead donor (the counter
                     // of a receiver is at least 1 here, so no borrow leaves it)
                     const uint32_t delta = dead ? (W_DEAD1 - 1u) : 0xFFFFFFFFu;
-                    const unsigned f = (word >> 16) & 0xFFu;
+                    unsigned f = (word >> 16) & 0xFFu;
                     int nxt = -1;
                     uint32_t nword = 0;
-#pragma unroll
-                    for (int d = 0; d < 8; d++) {
-                        if (!(f & (1u << ((d + 6) & 7)))) continue;   // fd code of direction d
+                    // receivers: the set bits of the fd code (bit b is direction (b + 2) & 7)
+                    while (f) {
+                        const int d = (__ffs(f) + 1) & 7;
+                        f &= f - 1;
                         int nr = lr + cdy(d), nc = lc + cdx(d);
                         if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
                             int rl = nr * TW + nc;
