@@ -125,8 +125,10 @@ def run_single(dem, precision, steps, warmup, tuning=None):
 
 def run_batched(ds, precision, steps, warmup, tuning=None):
     from parallem_amd import lemhip
-    cols = int(np.ceil(np.sqrt(len(ds))))
-    rows = int(np.ceil(len(ds) / cols))
+    # the grid of DEM rectangles with the fewest empty slots, then the squarest
+    n = len(ds)
+    cols = min(range(1, n + 1), key=lambda c: (c * -(-n // c), abs(c - -(-n // c))))
+    rows = -(-n // cols)
     aw, ah = cols * (SIDE + 1), rows * (SIDE + 1)
     atlas = lemhip.LemAtlas(lemhip.default_params(aw, ah, flowtype=0), aw, ah, precision, 0)
     slots = [atlas.add(d.z, d.rain, soilT_mode="nonneg") for d in ds]
