@@ -619,6 +619,7 @@ __device__ __forceinline__ void edge_insert_global(unsigned long long key, unsig
 // table once.  Pairs that do not find an LDS slot go to the global table
 // directly; the minimum is the same either way.
 #define EDGE_LDS 1024
+#define EDGE_ROWS 1
 __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__restrict__ mask, const int *__restrict__ ws,
                                                const double *__restrict__ z, const uint8_t *__restrict__ demid,
                                                unsigned long long *keys, unsigned long long *vals, long cap,
@@ -628,9 +629,11 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     for (int q = tid; q < EDGE_LDS; q += 256) { lk[q] = HASH_EMPTY; lv[q] = HASH_EMPTY; }
     __syncthreads();
-    int icol = blockIdx.x * blockDim.x + threadIdx.x;
-    int irow = blockIdx.y * blockDim.y + threadIdx.y;
-    long c = (long)irow * W + icol;
+    const int icol = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll 1
+    for (int rr = 0; rr < EDGE_ROWS; rr++) {
+    const int irow = (blockIdx.y * EDGE_ROWS + rr) * blockDim.y + threadIdx.y;
+    const long c = (long)irow * W + icol;
     if (icol < W && irow < H && mask[c]) {
         int me = ws[c];
         double zc = z[c];
@@ -662,6 +665,7 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
             if (!done) edge_insert_global(key, wenc, keys, vals, cap, overflow);
         }
     }
+    }
     __syncthreads();
     for (int q = tid; q < EDGE_LDS; q += 256) {
         unsigned long long key = lk[q];
@@ -685,9 +689,10 @@ __global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned 
             elo[slot] = lo;
             ehi[slot] = hi;
             ew[slot] = vals[i];
-            seen[lo] = 1;
-            seen[hi] = 1;
-            if (lo == 0) ds[labdem[hi]].zero_seen = 1;
+            // read before writing: label 0 and the slot flags are shared by many edges
+            if (!seen[lo]) seen[lo] = 1;
+            if (!seen[hi]) seen[hi] = 1;
+            if (lo == 0 && !ds[labdem[hi]].zero_seen) ds[labdem[hi]].zero_seen = 1;
         }
     }
 }
@@ -1151,7 +1156,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
         LEM_TRY(ensure_hash(c, want * 2));
         k_hash_clear<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals);
         LEM_CHECK(c, hipMemsetAsync(&c->scal->hash_overflow, 0, sizeof(int), c->s));
-        dim3 blk(64, 4), grd((W + 63) / 64, (H + 3) / 4);
+        dim3 blk(64, 4), grd((W + 63) / 64, (H + 4 * EDGE_ROWS - 1) / (4 * EDGE_ROWS));
         k_edges<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->demid, c->hkeys, c->hvals, c->hcap,
                                        &c->scal->hash_overflow);
         LEM_LAUNCHED(c);
