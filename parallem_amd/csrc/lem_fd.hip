@@ -266,6 +266,11 @@ __global__ void k_plat_first_list(int ntiles, const int *tile_has, int *list, Pl
     }
 }
 
+// threads per plateau workgroup: a launch near the end of the pass visits a
+// few tiles, each in K synchronised rounds over its region, so the rounds
+// are spread over many threads
+#define PLAT_NT 512
+
 // Distances live in two buffers per tile: tver[tile] = 2 * (launch that last
 // wrote the tile) + (buffer it wrote).  A launch reads every tile from the
 // buffer holding its state as of the previous launch and a visited tile
@@ -284,7 +289,7 @@ __global__ void k_plat_tver_init(int ntiles, int *tver)
 }
 
 template <int K, typename TS>
-__global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, int r0, const int *__restrict__ act,
+__global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch, int r0, const int *__restrict__ act,
                                                     int *__restrict__ next, float *__restrict__ sp0,
                                                     float *__restrict__ sp1, int *__restrict__ tver,
                                                     const uint8_t *__restrict__ eqb,
@@ -294,7 +299,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
                                                     PlatCtl *ctl)
 {
     constexpr int RW = PTW + 2 * K, RH = PTH + 2 * K, RN = RW * RH, NI = PTW * PTH;
-    constexpr int PER = (RN + 255) / 256, PERI = (NI + 255) / 256;
+    constexpr int PER = (RN + PLAT_NT - 1) / PLAT_NT, PERI = (NI + PLAT_NT - 1) / PLAT_NT;
     static_assert(PER <= 32, "region too large for the per-thread cell mask");
     __shared__ float A[RN], B[RN];
     __shared__ uint8_t E[RN];
@@ -321,7 +326,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         uint8_t e[PER];
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            int idx = threadIdx.x + q * 256;
+            int idx = threadIdx.x + q * PLAT_NT;
             int gy = ty0 - K + idx / RW, gx = tx0 - K + idx % RW;
             v[q] = 0.f;
             e[q] = 0;
@@ -334,7 +339,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         }
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            int idx = threadIdx.x + q * 256;
+            int idx = threadIdx.x + q * PLAT_NT;
             if (idx < RN) {
                 A[idx] = v[q];
                 B[idx] = v[q];
@@ -344,7 +349,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         }
 #pragma unroll
         for (int q = 0; q < PERI; q++) {
-            int li = threadIdx.x + q * 256;
+            int li = threadIdx.x + q * PLAT_NT;
             if (li < NI) { code[li] = 0; marks[li] = 0; }
         }
         __syncthreads();
@@ -353,7 +358,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         for (int k = 1; k <= K; k++) {
 #pragma unroll
             for (int q = 0; q < PER; q++) {
-                const int idx = threadIdx.x + q * 256;
+                const int idx = threadIdx.x + q * PLAT_NT;
                 const int rr = idx / RW, cc = idx % RW;
                 const bool act = ((own >> q) & 1) && rr >= k && rr < RH - k && cc >= k && cc < RW - k;
                 // the scan N..NW without branches; an inactive slot reads around
@@ -390,7 +395,7 @@ __global__ void __launch_bounds__(256) k_plat_block(int W, int H, int launch, in
         float *sp_out = s_buf[4] ? sp0 : sp1;
 #pragma unroll
         for (int q = 0; q < PERI; q++) {
-            int li = threadIdx.x + q * 256;
+            int li = threadIdx.x + q * PLAT_NT;
             int gy = ty0 + li / PTW, gx = tx0 + li % PTW;
             if (li >= NI || gy >= H || gx >= W) continue;
             long i = (long)gy * W + gx;
@@ -763,6 +768,52 @@ __global__ void k_minimax_relax(int ne, const uint32_t *elo, const uint32_t *ehi
     if (cb < rb) { atomicMin(&r[b], cb); *flag = 1; }
 }
 
+// The same pass with a bit per label that is set once the label's value is
+// finite (label 0 from the start): an edge between two labels the relaxation
+// has not reached yet can lower nothing, and the bits (one word per 32
+// labels) stay in the level-2 cache where the values do not.  A bit read
+// before another edge sets it in the same pass only defers that edge to the
+// next pass.
+__global__ void k_minimax_relax_reached(int ne, const uint32_t *elo, const uint32_t *ehi, const unsigned long long *ew,
+                                        unsigned long long *r, uint32_t *reached, int *flag)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    uint32_t a = elo[k], b = ehi[k];
+    bool ba = (reached[a >> 5] >> (a & 31)) & 1u, bb = (reached[b >> 5] >> (b & 31)) & 1u;
+    if (!ba && !bb) return;
+    unsigned long long w = ew[k];
+    unsigned long long ra = r[a], rb = r[b];
+    unsigned long long ca = w > rb ? w : rb;
+    unsigned long long cb = w > ra ? w : ra;
+    if (ca < ra) {
+        atomicMin(&r[a], ca);
+        if (!ba) atomicOr(&reached[a >> 5], 1u << (a & 31));
+        *flag = 1;
+    }
+    if (cb < rb) {
+        atomicMin(&r[b], cb);
+        if (!bb) atomicOr(&reached[b >> 5], 1u << (b & 31));
+        *flag = 1;
+    }
+}
+
+__global__ void k_iota_u32(int n, uint32_t *a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (uint32_t)i;
+}
+
+__global__ void k_edges_gather(int ne, const uint32_t *perm, const uint32_t *ehi, const unsigned long long *ew,
+                               uint32_t *hi2, unsigned long long *w2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    uint32_t k = perm[i];
+    hi2[i] = ehi[k];
+    w2[i] = ew[k];
+}
+
 // An edge with weight >= the values of both its labels can never lower
 // either (values only decrease and max(w, r) >= w); the relaxation drops such
 // edges between batches of passes and keeps iterating over the rest.
@@ -954,11 +1005,11 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     int bpc = 0;
     hipError_t oe = hipSuccess;
     switch (c->plat_k) {
-    case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<4, TS>, 256, 0); break;
-    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<16, TS>, 256, 0); break;
-    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<8, TS>, 256, 0); break;
+    case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<4, TS>, PLAT_NT, 0); break;
+    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<16, TS>, PLAT_NT, 0); break;
+    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<8, TS>, PLAT_NT, 0); break;
     }
-    if (oe != hipSuccess || bpc < 1) bpc = 4;
+    if (oe != hipSuccess || bpc < 1) bpc = 1;
     int G = bpc * c->ncu;
     if (G > ntiles) G = ntiles;
     const int BATCH = 8;
@@ -970,8 +1021,8 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
             int *a_next = L[(launch + 1) % 3];
             switch (c->plat_k) {
 #define PLAT_LAUNCH(K)                                                                                           \
-    k_plat_block<K, TS><<<G, 256, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, tver, eqb, c->fd, c->SFD, \
-                                             slopes, c->NP, tile_has, tstamp, ctl)
+    k_plat_block<K, TS><<<G, PLAT_NT, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, tver, eqb, c->fd,   \
+                                                 c->SFD, slopes, c->NP, tile_has, tstamp, ctl)
             case 4: PLAT_LAUNCH(4); break;
             case 16: PLAT_LAUNCH(16); break;
             default: PLAT_LAUNCH(8); break;
@@ -1177,17 +1228,49 @@ static int flooding(lemhip_ctx *c, int ncand1)
     LEM_TRY(read_scalars(c));
     int ne = c->hscal->nedges;
     c->h_nedges = ne;
+    if (ne > 1) {
+        // edges in label order: the relaxation passes then read the label
+        // values of neighbouring edges from the same cache lines (labels
+        // follow the sinks' row-major order, so adjacent basins have close
+        // labels).  Keys and permutation go through the hash tables, free
+        // after the compaction.
+        uint32_t *kin = c->elo, *kout = reinterpret_cast<uint32_t *>(c->hkeys);
+        uint32_t *pin = reinterpret_cast<uint32_t *>(c->hvals), *pout = kout + ne;
+        k_iota_u32<<<(ne + 255) / 256, 256, 0, c->s>>>(ne, pin);
+        LEM_LAUNCHED(c);
+        size_t need3 = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need3, kin, kout, pin, pout, ne, 0, 32, c->s);
+        if (need3 > c->cub_bytes) {
+            if (c->cub_tmp) (void)hipFree(c->cub_tmp);
+            c->cub_tmp = nullptr;
+            LEM_CHECK(c, hipMalloc(&c->cub_tmp, need3));
+            c->cub_bytes = need3;
+        }
+        LEM_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, need3, kin, kout, pin, pout, ne, 0, 32, c->s));
+        k_edges_gather<<<(ne + 255) / 256, 256, 0, c->s>>>(ne, pout, c->ehi, c->ew, c->elo2[0], c->ew2[0]);
+        LEM_LAUNCHED(c);
+        LEM_CHECK(c, hipMemcpyAsync(c->elo, kout, (size_t)ne * 4, hipMemcpyDeviceToDevice, c->s));
+        LEM_CHECK(c, hipMemcpyAsync(c->ehi, c->elo2[0], (size_t)ne * 4, hipMemcpyDeviceToDevice, c->s));
+        LEM_CHECK(c, hipMemcpyAsync(c->ew, c->ew2[0], (size_t)ne * 8, hipMemcpyDeviceToDevice, c->s));
+    }
 
     {
         // relaxation passes over a shrinking set of edges that can still lower a value
         const uint32_t *lo = c->elo, *hi = c->ehi;
         const unsigned long long *w = c->ew;
         int na = ne, buf = 0;
+        // reached-label bits: the component array is free until the cut-off fallback
+        uint32_t *reached = c->comp;
+        const long nw = ((long)nlab + 1 + 31) / 32;
+        LEM_CHECK(c, hipMemsetAsync(reached, 0, nw * sizeof(uint32_t), c->s));
+        c->hint[40] = 1;   // label 0
+        LEM_CHECK(c, hipMemcpyAsync(reached, &c->hint[40], sizeof(uint32_t), hipMemcpyHostToDevice, c->s));
         for (int it = 0;; it += 16) {
             LEM_TRY(clear_flag(c));
             if (na > 0)
                 for (int q = 0; q < 16; q++)
-                    k_minimax_relax<<<grid1d(na, 256, 1 << 30), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, &c->scal->flag);
+                    k_minimax_relax_reached<<<grid1d(na, 256, 1 << 30), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, reached,
+                                                                                         &c->scal->flag);
             LEM_LAUNCHED(c);
             uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
             unsigned long long *w2 = c->ew2[buf];
