@@ -78,6 +78,64 @@ __device__ __forceinline__ int block_append_n(int *counter, int n_mine, int *s_w
     return pos;
 }
 
+// Stream compaction in chunks of 256 * ITEMS elements (1-D blocks of 256):
+// one global atomic per chunk instead of one per 256 elements, since many
+// workgroups adding to one counter serialise.  Output order is a permutation
+// within each chunk.
+#define COMPACT_ITEMS 16
+template <int ITEMS, class Pred, class Emit>
+__device__ __forceinline__ void compact_chunks(long n, int *count, int *s_wave, int *s_base, Pred pred, Emit emit)
+{
+    const long CH = 256L * ITEMS;
+    for (long c0 = blockIdx.x * CH; c0 < n; c0 += (long)gridDim.x * CH) {
+        unsigned keep = 0;
+        int mine = 0;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            long i = c0 + j * 256 + threadIdx.x;
+            bool k = i < n && pred(i);
+            keep |= (unsigned)k << j;
+            mine += k;
+        }
+        int pos = block_append_n(count, mine, s_wave, s_base);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++)
+            if ((keep >> j) & 1u) emit(c0 + j * 256 + threadIdx.x, pos++);
+    }
+}
+
+// The same keeping the input order (each thread takes ITEMS consecutive
+// elements), for lists whose order carries locality.
+template <int ITEMS, class Pred, class Emit>
+__device__ __forceinline__ void compact_chunks_ordered(long n, int *count, int *s_wave, int *s_base, Pred pred,
+                                                       Emit emit)
+{
+    const long CH = 256L * ITEMS;
+    for (long c0 = blockIdx.x * CH; c0 < n; c0 += (long)gridDim.x * CH) {
+        const long t0 = c0 + (long)threadIdx.x * ITEMS;
+        unsigned keep = 0;
+        int mine = 0;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            bool k = t0 + j < n && pred(t0 + j);
+            keep |= (unsigned)k << j;
+            mine += k;
+        }
+        int pos = block_append_n(count, mine, s_wave, s_base);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++)
+            if ((keep >> j) & 1u) emit(t0 + j, pos++);
+    }
+}
+
+static inline int grid_chunks(long n, int cap = 8192)
+{
+    long g = (n + 256L * COMPACT_ITEMS - 1) / (256L * COMPACT_ITEMS);
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v, T *sh)
 {
@@ -546,12 +604,8 @@ __global__ void __launch_bounds__(256) k_jump_list(long N, const uint8_t *mask, 
 {
     __shared__ int s_wave[4];
     __shared__ int s_base;
-    for (long i0 = blockIdx.x * (long)blockDim.x; i0 < N; i0 += (long)gridDim.x * blockDim.x) {
-        long i = i0 + threadIdx.x;
-        bool keep = i < N && mask[i] && ws[i] < 0;
-        int at = block_append(count, keep, s_wave, &s_base);
-        if (keep) list[at] = (int)i;
-    }
+    compact_chunks<COMPACT_ITEMS>(N, count, s_wave, &s_base, [&](long i) { return mask[i] && ws[i] < 0; },
+                                  [&](long i, int at) { list[at] = (int)i; });
 }
 
 __global__ void k_jump_some(int n, const int *list, int *ws, int *flag)
@@ -685,11 +739,9 @@ __global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned 
 {
     __shared__ int s_wave[4];
     __shared__ int s_base;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < cap; i += (long)gridDim.x * blockDim.x) {
-        unsigned long long k = keys[i];
-        bool have = k != HASH_EMPTY;
-        int slot = block_append(count, have, s_wave, &s_base);
-        if (have) {
+    compact_chunks<COMPACT_ITEMS>(cap, count, s_wave, &s_base, [&](long i) { return keys[i] != HASH_EMPTY; },
+                                  [&](long i, int slot) {
+            unsigned long long k = keys[i];
             uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)(k & 0xffffffffu);
             elo[slot] = lo;
             ehi[slot] = hi;
@@ -698,8 +750,7 @@ __global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned 
             if (!seen[lo]) seen[lo] = 1;
             if (!seen[hi]) seen[hi] = 1;
             if (lo == 0 && !ds[labdem[hi]].zero_seen) ds[labdem[hi]].zero_seen = 1;
-        }
-    }
+        });
 }
 
 __global__ void k_count_bytes(long n, const uint8_t *a, int *out)
@@ -823,20 +874,9 @@ __global__ void __launch_bounds__(256) k_edges_keep(int ne, const uint32_t *elo,
 {
     __shared__ int s_wave[4];
     __shared__ int s_base;
-    for (int k0 = blockIdx.x * blockDim.x; k0 < ne; k0 += gridDim.x * blockDim.x) {
-        int k = k0 + threadIdx.x;
-        bool keep = false;
-        uint32_t a = 0, b = 0;
-        unsigned long long w = 0;
-        if (k < ne) {
-            a = elo[k];
-            b = ehi[k];
-            w = ew[k];
-            keep = w < r[a] || w < r[b];
-        }
-        int at = block_append(count, keep, s_wave, &s_base);
-        if (keep) { lo2[at] = a; hi2[at] = b; w2[at] = w; }
-    }
+    compact_chunks_ordered<COMPACT_ITEMS>(
+        ne, count, s_wave, &s_base, [&](long k) { unsigned long long w = ew[k]; return w < r[elo[k]] || w < r[ehi[k]]; },
+        [&](long k, int at) { lo2[at] = elo[k]; hi2[at] = ehi[k]; w2[at] = ew[k]; });
 }
 
 // Basins with no path to label 0 (only possible when NaN heights cut them
@@ -1183,7 +1223,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
         if (!f) break;
         int *jlist = c->scan_tmp;
         LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
-        k_jump_list<<<grid1d(N, 256, 8192), 256, 0, c->s>>>(N, c->mask, c->ws, jlist, &c->scal->ncand);
+        k_jump_list<<<grid_chunks(N), 256, 0, c->s>>>(N, c->mask, c->ws, jlist, &c->scal->ncand);
         LEM_LAUNCHED(c);
         LEM_TRY(read_scalars(c));
         int nj = c->hscal->ncand;
@@ -1221,7 +1261,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     k_minimax_init<<<grid1d(nlab + 1), 256, 0, c->s>>>(nlab + 1, c->rmm, c->seen);
     LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
     k_slots_sinkcount_init<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->dsc);
-    k_edges_compact<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals, c->elo, c->ehi, c->ew,
+    k_edges_compact<<<grid_chunks(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals, c->elo, c->ehi, c->ew,
                                                        &c->scal->nedges, c->seen, c->labdem, c->dsc);
     k_count_labels<<<grid1d(nlab + 1, 256, 1024), 256, 0, c->s>>>(nlab + 1, c->seen, c->labdem, c->nslot, c->dsc);
     LEM_LAUNCHED(c);
@@ -1276,7 +1316,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
             unsigned long long *w2 = c->ew2[buf];
             LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
             if (na > 0)
-                k_edges_keep<<<grid1d(na, 256, 8192), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, lo2, hi2, w2,
+                k_edges_keep<<<grid_chunks(na), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, lo2, hi2, w2,
                                                                       &c->scal->nedges);
             LEM_LAUNCHED(c);
             LEM_TRY(read_scalars(c));
