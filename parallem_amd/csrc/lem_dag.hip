@@ -229,28 +229,49 @@ struct SedCell {
 
 // ------------------------------------------------------------ prep kernels
 
+// Both prep kernels read each cell's 3x3 neighbourhood.  A 64 x 16 block of
+// cells stages its 66 x 18 region (rows coalesced) in LDS once; off-grid
+// positions are staged as nodata, which is how every rule treats them (D11).
+#define PREP_RG 4   // rows per thread: the block covers 64 x (4 * PREP_RG) cells
+#define PREP_RW 66
+#define PREP_RH (4 * PREP_RG + 2)
+
 // donor bits in pull order; status: FA computes every mask cell, nodata and
 // off-grid neighbours are complete from the start and contribute nothing
-__global__ void k_fa_prep(int W, int H, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ fd,
-                          uint8_t *__restrict__ dm, int *__restrict__ st)
+__global__ void __launch_bounds__(256) k_fa_prep(int W, int H, const uint8_t *__restrict__ mask,
+                                                 const uint8_t *__restrict__ fd, uint8_t *__restrict__ dm,
+                                                 int *__restrict__ st)
 {
-    int icol = blockIdx.x * blockDim.x + threadIdx.x;
-    int irow = blockIdx.y * blockDim.y + threadIdx.y;
-    if (icol >= W || irow >= H) return;
-    long i = (long)irow * W + icol;
-    unsigned bits = 0;
-    if (mask[i]) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            int d = kPULL[k];
-            int cx = icol + kDX[d], cy = irow + kDY[d];
-            if (cx < 0 || cx >= W || cy < 0 || cy >= H) continue;
-            long n = (long)cy * W + cx;
-            if (mask[n] && (fd[n] & donor_code(d))) bits |= 1u << k;
-        }
+    __shared__ uint8_t smk[PREP_RH * PREP_RW], sfd[PREP_RH * PREP_RW];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * (4 * PREP_RG);
+    for (int q = tid; q < PREP_RH * PREP_RW; q += 256) {
+        int gy = y0 - 1 + q / PREP_RW, gx = x0 - 1 + q % PREP_RW;
+        bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+        long n = (long)gy * W + gx;
+        smk[q] = in ? mask[n] : 0;
+        sfd[q] = in ? fd[n] : 0;
     }
-    dm[i] = (uint8_t)bits;
-    st[i] = mask[i] ? DAG_PENDING : 0;
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < PREP_RG; g++) {
+        const int lr = g * 4 + threadIdx.y;
+        const int icol = x0 + threadIdx.x, irow = y0 + lr;
+        if (icol >= W || irow >= H) continue;
+        const long i = (long)irow * W + icol;
+        const int p = (lr + 1) * PREP_RW + threadIdx.x + 1;
+        unsigned bits = 0;
+        if (smk[p]) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int d = cpull(k);
+                const int pn = p + cdy(d) * PREP_RW + cdx(d);
+                if (smk[pn] && (sfd[pn] & donor_code(d))) bits |= 1u << k;
+            }
+        }
+        dm[i] = (uint8_t)bits;
+        st[i] = smk[p] ? DAG_PENDING : 0;
+    }
 }
 
 // MFDsedbud never computes row 0 / column 0 (mfd_accum.cu:42) and nodata
@@ -261,51 +282,69 @@ __global__ void k_fa_prep(int W, int H, const uint8_t *__restrict__ mask, const 
 // incision and gelifluction terms in the order E..NE plus the cell's own, each
 // weighted by its deposition fraction; it reads only inputs of the pass.
 template <typename T>
-__global__ void k_sed_prep(int W, int H, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ fd,
-                           const uint8_t *__restrict__ demid, const DemDesc *__restrict__ dd,
-                           uint8_t *__restrict__ dm, int *__restrict__ st, const T *__restrict__ ero,
-                           const T *__restrict__ inci, const T *__restrict__ geli, double ddk, double dck, double dgk,
-                           T *__restrict__ dps)
+__global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *__restrict__ mask,
+                                                  const uint8_t *__restrict__ fd, const uint8_t *__restrict__ demid,
+                                                  const DemDesc *__restrict__ dd, uint8_t *__restrict__ dm,
+                                                  int *__restrict__ st, const T *__restrict__ ero,
+                                                  const T *__restrict__ inci, const T *__restrict__ geli, double ddk,
+                                                  double dck, double dgk, T *__restrict__ dps)
 {
-    int icol = blockIdx.x * blockDim.x + threadIdx.x;
-    int irow = blockIdx.y * blockDim.y + threadIdx.y;
-    if (icol >= W || irow >= H) return;
-    long i = (long)irow * W + icol;
-    unsigned bits = 0;
-    int s = 0;
-    if (mask[i]) {
-        int lr = irow, lc = icol;
-        if (demid) { DemDesc D = dd[demid[i]]; lr -= D.y0; lc -= D.x0; }
-        bool dead = (lr == 0 || lc == 0);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            int d = kPULL[k];
-            int cx = icol + kDX[d], cy = irow + kDY[d];
-            bool off = (cx < 0 || cx >= W || cy < 0 || cy >= H);
-            long n = (long)cy * W + cx;
-            if (off || !mask[n]) {
-                if ((-9999) & donor_code(d)) dead = true;
-                continue;
-            }
-            if (fd[n] & donor_code(d)) bits |= 1u << k;
-        }
-        s = dead ? -1 : DAG_PENDING;
-        T accum_d = 0, accum_c = 0, accum_g = 0;
-        const T one = (T)1;
-        const T kd = (T)(1 - ddk), kc = (T)(1 - dck), kg = (T)(1 - dgk);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            if (!(bits & (1u << k))) continue;
-            int d = kPULL[k];
-            long n = (long)(irow + kDY[d]) * W + (icol + kDX[d]);
-            accum_d += one * (ero[n] * kd);
-            accum_c += one * (inci[n] * kc);
-            accum_g += one * (geli[n] * kg);
-        }
-        dps[i] = ((accum_d + ero[i]) * (T)ddk) + ((accum_c + inci[i]) * (T)dck) + ((accum_g + geli[i]) * (T)dgk);
+    __shared__ uint8_t smk[PREP_RH * PREP_RW], sfd[PREP_RH * PREP_RW];
+    __shared__ T se[PREP_RH * PREP_RW], si[PREP_RH * PREP_RW], sg[PREP_RH * PREP_RW];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * (4 * PREP_RG);
+    for (int q = tid; q < PREP_RH * PREP_RW; q += 256) {
+        int gy = y0 - 1 + q / PREP_RW, gx = x0 - 1 + q % PREP_RW;
+        bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+        long n = (long)gy * W + gx;
+        uint8_t m = in ? mask[n] : 0;
+        smk[q] = m;
+        sfd[q] = in ? fd[n] : 0;
+        se[q] = m ? ero[n] : (T)0;
+        si[q] = m ? inci[n] : (T)0;
+        sg[q] = m ? geli[n] : (T)0;
     }
-    dm[i] = (uint8_t)bits;
-    st[i] = s;
+    __syncthreads();
+    const T one = (T)1;
+    const T kd = (T)(1 - ddk), kc = (T)(1 - dck), kg = (T)(1 - dgk);
+#pragma unroll
+    for (int g = 0; g < PREP_RG; g++) {
+        const int lr = g * 4 + threadIdx.y;
+        const int icol = x0 + threadIdx.x, irow = y0 + lr;
+        if (icol >= W || irow >= H) continue;
+        const long i = (long)irow * W + icol;
+        const int p = (lr + 1) * PREP_RW + threadIdx.x + 1;
+        unsigned bits = 0;
+        int s = 0;
+        if (smk[p]) {
+            int dr = irow, dc = icol;
+            if (demid) { DemDesc D = dd[demid[i]]; dr -= D.y0; dc -= D.x0; }
+            bool dead = (dr == 0 || dc == 0);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int d = cpull(k);
+                const int pn = p + cdy(d) * PREP_RW + cdx(d);
+                if (!smk[pn]) {
+                    if ((-9999) & donor_code(d)) dead = true;
+                    continue;
+                }
+                if (sfd[pn] & donor_code(d)) bits |= 1u << k;
+            }
+            s = dead ? -1 : DAG_PENDING;
+            T accum_d = 0, accum_c = 0, accum_g = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (!(bits & (1u << k))) continue;
+                const int pn = p + cdy(cpull(k)) * PREP_RW + cdx(cpull(k));
+                accum_d += one * (se[pn] * kd);
+                accum_c += one * (si[pn] * kc);
+                accum_g += one * (sg[pn] * kg);
+            }
+            dps[i] = ((accum_d + se[p]) * (T)ddk) + ((accum_c + si[p]) * (T)dck) + ((accum_g + sg[p]) * (T)dgk);
+        }
+        dm[i] = (uint8_t)bits;
+        st[i] = s;
+    }
 }
 
 // ------------------------------------------------------------ tiled rounds
@@ -686,7 +725,7 @@ static int sed_shape(lemhip_ctx *c)
 template <typename T>
 int lem_fa_dag(lemhip_ctx *c)
 {
-    dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 3) / 4);
+    dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 4 * PREP_RG - 1) / (4 * PREP_RG));
     int *st = reinterpret_cast<int *>(c->scan_tmp);
     k_fa_prep<<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->dmask, st);
     LEM_LAUNCHED(c);
@@ -701,7 +740,7 @@ int lem_fa_dag(lemhip_ctx *c)
 template <typename T>
 int lem_sed_dag(lemhip_ctx *c)
 {
-    dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 3) / 4);
+    dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 4 * PREP_RG - 1) / (4 * PREP_RG));
     int *st = reinterpret_cast<int *>(c->scan_tmp);
     double ddk = c->p.ddk, dck = c->p.dck, dgk = c->p.dgk;
     if (ddk > 0.95) ddk = 0.95;   // mfd_accum.cu:406-407
