@@ -639,14 +639,6 @@ __global__ void k_jump(long N, const uint8_t *mask, int *ws, int *flag)
 
 #define HASH_EMPTY 0xFFFFFFFFFFFFFFFFULL
 
-__global__ void k_hash_clear(long cap, unsigned long long *keys, unsigned long long *vals)
-{
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < cap; i += (long)gridDim.x * blockDim.x) {
-        keys[i] = HASH_EMPTY;
-        vals[i] = HASH_EMPTY;
-    }
-}
-
 __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 {
     k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
@@ -655,43 +647,29 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 
 // identify_watershed_boundary_write_backMFD (newflow.cu:787-877) without the
 // fixed table (D4): every mask cell c and in-grid neighbour n with different
-// labels and z_c >= z_n contributes edge {ws_c, ws_n} at height z_c; the
-// table keeps the minimum per unordered pair (order-free, exact).
-__device__ __forceinline__ void edge_insert_global(unsigned long long key, unsigned long long wenc,
-                                                   unsigned long long *keys, unsigned long long *vals, long cap,
-                                                   int *overflow)
-{
-    unsigned long long m = (unsigned long long)(cap - 1);
-    unsigned long long h = mix64(key) & m;
-    long probes = 0;
-    while (true) {
-        unsigned long long prev = atomicCAS(&keys[h], HASH_EMPTY, key);
-        if (prev == HASH_EMPTY || prev == key) { atomicMin(&vals[h], wenc); break; }
-        h = (h + 1) & m;
-        if (++probes >= cap) { *overflow = 1; break; }
-    }
-}
-
-// The cells of a block share most of their basin pairs (a boundary runs
-// through many cells): pairs are first merged in an LDS table (minimum weight
-// per pair, exact), and each distinct pair of the block goes to the global
-// table once.  Pairs that do not find an LDS slot go to the global table
-// directly; the minimum is the same either way.
+// labels and z_c >= z_n contributes edge {ws_c, ws_n} at height z_c, and the
+// edge list keeps the minimum per unordered pair (order-free, exact).  The
+// cells of a block share most of their basin pairs (a boundary runs through
+// many cells): pairs are first merged in an LDS table (minimum weight per
+// pair), and each block appends its distinct pairs to a list with one atomic;
+// pairs that find no LDS slot are appended one by one.  The list is then
+// sorted by pair and reduced to the minimum weight per pair.  Keys pack
+// (lo << B) | hi with B bits per label.  The count may exceed the capacity:
+// the caller then grows the list and runs again.
 #define EDGE_LDS 1024
-#define EDGE_ROWS 1
-__global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__restrict__ mask, const int *__restrict__ ws,
-                                               const double *__restrict__ z, const uint8_t *__restrict__ demid,
-                                               unsigned long long *keys, unsigned long long *vals, long cap,
-                                               int *overflow)
+__global__ void __launch_bounds__(256) k_edges_emit(int W, int H, const uint8_t *__restrict__ mask,
+                                                    const int *__restrict__ ws, const double *__restrict__ z,
+                                                    const uint8_t *__restrict__ demid, unsigned long long *keys,
+                                                    unsigned long long *vals, long cap, int *count, int B)
 {
     __shared__ unsigned long long lk[EDGE_LDS], lv[EDGE_LDS];
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     for (int q = tid; q < EDGE_LDS; q += 256) { lk[q] = HASH_EMPTY; lv[q] = HASH_EMPTY; }
     __syncthreads();
     const int icol = blockIdx.x * blockDim.x + threadIdx.x;
-#pragma unroll 1
-    for (int rr = 0; rr < EDGE_ROWS; rr++) {
-    const int irow = (blockIdx.y * EDGE_ROWS + rr) * blockDim.y + threadIdx.y;
+    const int irow = blockIdx.y * blockDim.y + threadIdx.y;
     const long c = (long)irow * W + icol;
     if (icol < W && irow < H && mask[c]) {
         int me = ws[c];
@@ -712,8 +690,8 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
             for (int q = 0; q < ns; q++) dup |= (seen[q] == other);
             if (dup) continue;
             seen[ns++] = other;
-            unsigned lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
-            unsigned long long key = ((unsigned long long)lo << 32) | hi;
+            unsigned long long lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
+            unsigned long long key = (lo << B) | hi;
             unsigned h = (unsigned)(mix64(key) & (EDGE_LDS - 1));
             bool done = false;
             for (int p = 0; p < 16 && !done; p++) {
@@ -721,36 +699,42 @@ __global__ void __launch_bounds__(256) k_edges(int W, int H, const uint8_t *__re
                 if (prev == HASH_EMPTY || prev == key) { atomicMin(&lv[h], wenc); done = true; }
                 h = (h + 1) & (EDGE_LDS - 1);
             }
-            if (!done) edge_insert_global(key, wenc, keys, vals, cap, overflow);
+            if (!done) {
+                int at = atomicAdd(count, 1);
+                if (at < cap) { keys[at] = key; vals[at] = wenc; }
+            }
         }
     }
-    }
     __syncthreads();
-    for (int q = tid; q < EDGE_LDS; q += 256) {
-        unsigned long long key = lk[q];
-        if (key != HASH_EMPTY) edge_insert_global(key, lv[q], keys, vals, cap, overflow);
+    constexpr int PERT = EDGE_LDS / 256;
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < PERT; j++) mine += lk[tid * PERT + j] != HASH_EMPTY;
+    int pos = block_append_n(count, mine, s_wave, &s_base);
+#pragma unroll
+    for (int j = 0; j < PERT; j++) {
+        unsigned long long key = lk[tid * PERT + j];
+        if (key == HASH_EMPTY) continue;
+        if (pos < cap) { keys[pos] = key; vals[pos] = lv[tid * PERT + j]; }
+        pos++;
     }
 }
 
-__global__ void __launch_bounds__(256) k_edges_compact(long cap, const unsigned long long *keys,
-                                                       const unsigned long long *vals, uint32_t *elo, uint32_t *ehi,
-                                                       unsigned long long *ew, int *count, uint8_t *seen,
-                                                       const uint8_t *labdem, DemScal *ds)
+// unique pairs (sorted) to the edge list; label flags as in the compaction
+__global__ void k_edges_decode(int ne, const unsigned long long *ukeys, const unsigned long long *uw, int B,
+                               uint32_t *elo, uint32_t *ehi, unsigned long long *ew, uint8_t *seen,
+                               const uint8_t *labdem, DemScal *ds)
 {
-    __shared__ int s_wave[4];
-    __shared__ int s_base;
-    compact_chunks<COMPACT_ITEMS>(cap, count, s_wave, &s_base, [&](long i) { return keys[i] != HASH_EMPTY; },
-                                  [&](long i, int slot) {
-            unsigned long long k = keys[i];
-            uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)(k & 0xffffffffu);
-            elo[slot] = lo;
-            ehi[slot] = hi;
-            ew[slot] = vals[i];
-            // read before writing: label 0 and the slot flags are shared by many edges
-            if (!seen[lo]) seen[lo] = 1;
-            if (!seen[hi]) seen[hi] = 1;
-            if (lo == 0 && !ds[labdem[hi]].zero_seen) ds[labdem[hi]].zero_seen = 1;
-        });
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    unsigned long long key = ukeys[k];
+    uint32_t lo = (uint32_t)(key >> B), hi = (uint32_t)(key & ((1ULL << B) - 1));
+    elo[k] = lo;
+    ehi[k] = hi;
+    ew[k] = uw[k];
+    if (!seen[lo]) seen[lo] = 1;
+    if (!seen[hi]) seen[hi] = 1;
+    if (lo == 0 && !ds[labdem[hi]].zero_seen) ds[labdem[hi]].zero_seen = 1;
 }
 
 __global__ void k_count_bytes(long n, const uint8_t *a, int *out)
@@ -1238,61 +1222,61 @@ static int flooding(lemhip_ctx *c, int ncand1)
         }
         break;
     }
-    // edges
+    // edges: block-distinct pairs appended, sorted by pair, reduced to the
+    // minimum weight per pair; the list comes out sorted by label, so the
+    // relaxation passes read neighbouring labels' values from the same lines
     LEM_TRY(ensure_labels(c, (long)nlab + 1));
-    long want = 4L * (nlab + 1);   // about three edges per basin
-    if (want > 4 * N) want = 4 * N;
+    int B = 1;
+    while ((1L << B) <= (long)nlab) B++;
+    long want = 8L * (nlab + 1);
     if (want < 1024) want = 1024;
+    long np = 0;
     for (int attempt = 0;; ++attempt) {
-        LEM_TRY(ensure_hash(c, want * 2));
-        k_hash_clear<<<grid1d(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals);
-        LEM_CHECK(c, hipMemsetAsync(&c->scal->hash_overflow, 0, sizeof(int), c->s));
-        dim3 blk(64, 4), grd((W + 63) / 64, (H + 4 * EDGE_ROWS - 1) / (4 * EDGE_ROWS));
-        k_edges<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->demid, c->hkeys, c->hvals, c->hcap,
-                                       &c->scal->hash_overflow);
+        LEM_TRY(ensure_hash(c, want));
+        LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
+        dim3 blk(64, 4), grd((W + 63) / 64, (H + 3) / 4);
+        k_edges_emit<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->demid, c->hkeys, c->hvals, c->hcap,
+                                            &c->scal->nedges, B);
         LEM_LAUNCHED(c);
-        LEM_CHECK(c, hipMemcpyAsync(&c->hscal->hash_overflow, &c->scal->hash_overflow, sizeof(int),
-                                    hipMemcpyDeviceToHost, c->s));
-        LEM_CHECK(c, hipStreamSynchronize(c->s));
-        if (!c->hscal->hash_overflow) break;
-        want = c->hcap * 2;
-        if (attempt > 8) { c->err = "edge table overflow"; return LEMHIP_E_NOMEM; }
+        LEM_TRY(read_scalars(c));
+        np = c->hscal->nedges;
+        if (np <= c->hcap) break;
+        want = np + np / 4;
+        if (attempt > 8) { c->err = "edge list overflow"; return LEMHIP_E_NOMEM; }
     }
     k_minimax_init<<<grid1d(nlab + 1), 256, 0, c->s>>>(nlab + 1, c->rmm, c->seen);
-    LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
     k_slots_sinkcount_init<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->dsc);
-    k_edges_compact<<<grid_chunks(c->hcap), 256, 0, c->s>>>(c->hcap, c->hkeys, c->hvals, c->elo, c->ehi, c->ew,
-                                                       &c->scal->nedges, c->seen, c->labdem, c->dsc);
-    k_count_labels<<<grid1d(nlab + 1, 256, 1024), 256, 0, c->s>>>(nlab + 1, c->seen, c->labdem, c->nslot, c->dsc);
     LEM_LAUNCHED(c);
-    LEM_TRY(read_scalars(c));
-    int ne = c->hscal->nedges;
-    c->h_nedges = ne;
-    if (ne > 1) {
-        // edges in label order: the relaxation passes then read the label
-        // values of neighbouring edges from the same cache lines (labels
-        // follow the sinks' row-major order, so adjacent basins have close
-        // labels).  Keys and permutation go through the hash tables, free
-        // after the compaction.
-        uint32_t *kin = c->elo, *kout = reinterpret_cast<uint32_t *>(c->hkeys);
-        uint32_t *pin = reinterpret_cast<uint32_t *>(c->hvals), *pout = kout + ne;
-        k_iota_u32<<<(ne + 255) / 256, 256, 0, c->s>>>(ne, pin);
-        LEM_LAUNCHED(c);
-        size_t need3 = 0;
-        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need3, kin, kout, pin, pout, ne, 0, 32, c->s);
+    int ne = 0;
+    if (np > 0) {
+        unsigned long long *skeys = c->ew2[0], *svals = c->ew2[1];
+        size_t need3 = 0, need4 = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need3, c->hkeys, skeys, c->hvals, svals, (int)np, 0, 2 * B,
+                                                 c->s);
+        (void)hipcub::DeviceReduce::ReduceByKey(nullptr, need4, skeys, c->hkeys, svals, c->hvals, &c->scal->nedges,
+                                                hipcub::Min(), (int)np, c->s);
+        if (need4 > need3) need3 = need4;
         if (need3 > c->cub_bytes) {
             if (c->cub_tmp) (void)hipFree(c->cub_tmp);
             c->cub_tmp = nullptr;
             LEM_CHECK(c, hipMalloc(&c->cub_tmp, need3));
             c->cub_bytes = need3;
         }
-        LEM_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, need3, kin, kout, pin, pout, ne, 0, 32, c->s));
-        k_edges_gather<<<(ne + 255) / 256, 256, 0, c->s>>>(ne, pout, c->ehi, c->ew, c->elo2[0], c->ew2[0]);
+        size_t t3 = c->cub_bytes;
+        LEM_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, t3, c->hkeys, skeys, c->hvals, svals, (int)np, 0,
+                                                         2 * B, c->s));
+        t3 = c->cub_bytes;
+        LEM_CHECK(c, hipcub::DeviceReduce::ReduceByKey(c->cub_tmp, t3, skeys, c->hkeys, svals, c->hvals,
+                                                        &c->scal->nedges, hipcub::Min(), (int)np, c->s));
+        LEM_TRY(read_scalars(c));
+        ne = c->hscal->nedges;
+        k_edges_decode<<<(ne + 255) / 256, 256, 0, c->s>>>(ne, c->hkeys, c->hvals, B, c->elo, c->ehi, c->ew, c->seen,
+                                                          c->labdem, c->dsc);
         LEM_LAUNCHED(c);
-        LEM_CHECK(c, hipMemcpyAsync(c->elo, kout, (size_t)ne * 4, hipMemcpyDeviceToDevice, c->s));
-        LEM_CHECK(c, hipMemcpyAsync(c->ehi, c->elo2[0], (size_t)ne * 4, hipMemcpyDeviceToDevice, c->s));
-        LEM_CHECK(c, hipMemcpyAsync(c->ew, c->ew2[0], (size_t)ne * 8, hipMemcpyDeviceToDevice, c->s));
     }
+    k_count_labels<<<grid1d(nlab + 1, 256, 1024), 256, 0, c->s>>>(nlab + 1, c->seen, c->labdem, c->nslot, c->dsc);
+    LEM_LAUNCHED(c);
+    c->h_nedges = ne;
 
     {
         // relaxation passes over a shrinking set of edges that can still lower a value
