@@ -98,6 +98,7 @@ struct FaCell {
     static constexpr int NC = TW * TH;
     static constexpr int NP = (TW + 2) * (TH + 2);
     static constexpr bool kCanDie = false;
+    static constexpr int kWaves = 6;   // waves per SIMD the register budget is cut for
     const T *w;
     FS *fa;
     struct Lds {
@@ -156,6 +157,7 @@ struct SedCell {
     static constexpr int NC = TW * TH;
     static constexpr int NP = (TW + 2) * (TH + 2);
     static constexpr bool kCanDie = true;
+    static constexpr int kWaves = 5;
     const T *ero, *inci, *geli, *dps;
     T *depo, *stone, *fines;
     double *soilT;
@@ -380,7 +382,7 @@ __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)
 // its donors' results (the cell body's pre-phase) is done for all cells of
 // the tile in parallel before the walks.
 template <int TW, int TH, class Cell>
-__global__ void __launch_bounds__(256) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
+__global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
                                                    const uint8_t *__restrict__ fdv, int *__restrict__ st,
                                                    int *__restrict__ tile_done, const int *__restrict__ act,
                                                    int *__restrict__ next, int *__restrict__ wake, DagCtl *ctl)
