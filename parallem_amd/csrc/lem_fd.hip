@@ -737,15 +737,6 @@ __global__ void k_edges_decode(int ne, const unsigned long long *ukeys, const un
     if (lo == 0 && !ds[labdem[hi]].zero_seen) ds[labdem[hi]].zero_seen = 1;
 }
 
-__global__ void k_count_bytes(long n, const uint8_t *a, int *out)
-{
-    __shared__ int sh[256];
-    int acc = 0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) acc += a[i];
-    acc = block_reduce_sum<int>(acc, sh);
-    if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
-}
-
 // distinct labels of each slot in the edge set (newflow.cu:1045: data->sinkcount)
 __global__ void __launch_bounds__(256) k_count_labels(long n, const uint8_t *seen, const uint8_t *labdem, int ns,
                                                       DemScal *ds)
@@ -831,22 +822,6 @@ __global__ void k_minimax_relax_reached(int ne, const uint32_t *elo, const uint3
         if (!bb) atomicOr(&reached[b >> 5], 1u << (b & 31));
         *flag = 1;
     }
-}
-
-__global__ void k_iota_u32(int n, uint32_t *a)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = (uint32_t)i;
-}
-
-__global__ void k_edges_gather(int ne, const uint32_t *perm, const uint32_t *ehi, const unsigned long long *ew,
-                               uint32_t *hi2, unsigned long long *w2)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ne) return;
-    uint32_t k = perm[i];
-    hi2[i] = ehi[k];
-    w2[i] = ew[k];
 }
 
 // An edge with weight >= the values of both its labels can never lower
