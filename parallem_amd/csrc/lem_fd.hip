@@ -104,27 +104,47 @@ __device__ __forceinline__ void compact_chunks(long n, int *count, int *s_wave, 
     }
 }
 
-// The same keeping the input order (each thread takes ITEMS consecutive
-// elements), for lists whose order carries locality.
+// The same keeping the input order: the chunk is read row by row (256
+// consecutive elements, coalesced); one atomic reserves the chunk's output
+// range, then each row's kept elements take their ranks from a block-wide
+// scan of that row.
 template <int ITEMS, class Pred, class Emit>
 __device__ __forceinline__ void compact_chunks_ordered(long n, int *count, int *s_wave, int *s_base, Pred pred,
                                                        Emit emit)
 {
+    __shared__ int s_row[ITEMS][4];
     const long CH = 256L * ITEMS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (long c0 = blockIdx.x * CH; c0 < n; c0 += (long)gridDim.x * CH) {
-        const long t0 = c0 + (long)threadIdx.x * ITEMS;
         unsigned keep = 0;
-        int mine = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
-            bool k = t0 + j < n && pred(t0 + j);
+            long i = c0 + j * 256 + threadIdx.x;
+            bool k = i < n && pred(i);
             keep |= (unsigned)k << j;
-            mine += k;
+            unsigned long long b = __ballot(k);
+            if (lane == 0) s_row[j][wave] = __popcll(b);
         }
-        int pos = block_append_n(count, mine, s_wave, s_base);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int j = 0; j < ITEMS; j++)
+                for (int w = 0; w < 4; w++) {
+                    int c = s_row[j][w];
+                    s_row[j][w] = tot;
+                    tot += c;
+                }
+            *s_base = tot ? atomicAdd(count, tot) : 0;
+        }
+        __syncthreads();
+        const int base = *s_base;
 #pragma unroll
-        for (int j = 0; j < ITEMS; j++)
-            if ((keep >> j) & 1u) emit(t0 + j, pos++);
+        for (int j = 0; j < ITEMS; j++) {
+            bool k = (keep >> j) & 1u;
+            unsigned long long b = __ballot(k);
+            if (k) emit(c0 + j * 256 + threadIdx.x, base + s_row[j][wave] + __popcll(b & ((1ULL << lane) - 1ULL)));
+        }
+        __syncthreads();
     }
 }
 
