@@ -433,15 +433,19 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
         __syncthreads();
         float *cur = A, *nxt = B;
         int moved = 0;   // last round in which one of this thread's tile cells changed
+        // A round in which no cell of its window changes leaves every later
+        // round unchanged too (round k+1 reads only round k values of a window
+        // that contains its own), so the launch stops there.
         for (int k = 1; k <= K; k++) {
+            bool changed = false;
 #pragma unroll
             for (int q = 0; q < PER; q++) {
                 const int idx = threadIdx.x + q * PLAT_NT;
                 const int rr = idx / RW, cc = idx % RW;
                 const bool act = ((own >> q) & 1) && rr >= k && rr < RH - k && cc >= k && cc < RW - k;
-                // the scan N..NW without branches; an inactive slot reads around
-                // a fixed interior cell and discards the result
-                const int b = act ? idx : RW + 1;
+                if (!act) continue;
+                // the scan N..NW without branches
+                const int b = idx;
                 float md = cur[b];
                 const unsigned ee = E[b];   // read per round: keeps 8 lane masks per cell out of the registers
                 unsigned cd = 0, mk = 0;
@@ -454,18 +458,18 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
                     cd = take ? (1u << ((d + 6) & 7)) : cd;   // fd code of direction d
                     mk |= take ? (1u << d) : 0u;
                 }
-                if (act) {
-                    nxt[idx] = md;
-                    if (cd && rr >= K && rr < K + PTH && cc >= K && cc < K + PTW) {
-                        int li = (rr - K) * PTW + (cc - K);
-                        code[li] = (uint8_t)cd;
-                        marks[li] |= (uint8_t)mk;
-                        moved = k;
-                    }
+                nxt[idx] = md;
+                changed |= cd != 0;
+                if (cd && rr >= K && rr < K + PTH && cc >= K && cc < K + PTW) {
+                    int li = (rr - K) * PTW + (cc - K);
+                    code[li] = (uint8_t)cd;
+                    marks[li] |= (uint8_t)mk;
+                    moved = k;
                 }
             }
-            __syncthreads();
+            const int any = __syncthreads_or(changed);
             float *t = cur; cur = nxt; nxt = t;
+            if (!any) break;
         }
         if (moved) atomicMax(&last, moved);
         // write back the tile: distances (into its other buffer), directions of
