@@ -366,7 +366,9 @@ __global__ void k_plat_tver_init(int ntiles, int *tver)
     if (t < ntiles) tver[t] = -2;   // buffer 0, written before any launch
 }
 
-template <int K, typename TS>
+// MARKS: the pass records slope marks (the first pass, on the dem; the pass
+// on the raised surface keeps only directions and distances)
+template <int K, typename TS, bool MARKS>
 __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch, int r0, const int *__restrict__ act,
                                                     int *__restrict__ next, float *__restrict__ sp0,
                                                     float *__restrict__ sp1, int *__restrict__ tver,
@@ -456,14 +458,14 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
                     const bool take = ((ee >> d) & 1u) && (md > dis);
                     md = take ? dis : md;
                     cd = take ? (1u << ((d + 6) & 7)) : cd;   // fd code of direction d
-                    mk |= take ? (1u << d) : 0u;
+                    if (MARKS) mk |= take ? (1u << d) : 0u;
                 }
                 nxt[idx] = md;
                 changed |= cd != 0;
                 if (cd && rr >= K && rr < K + PTH && cc >= K && cc < K + PTW) {
                     int li = (rr - K) * PTW + (cc - K);
                     code[li] = (uint8_t)cd;
-                    marks[li] |= (uint8_t)mk;
+                    if (MARKS) marks[li] |= (uint8_t)mk;
                     moved = k;
                 }
             }
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
             sp_out[i] = cur[(li / PTW + K) * RW + (li % PTW + K)];
             uint8_t cd = code[li];
             if (cd) { fd[i] = cd; SFD[i] = cd; }
-            if (slopes && marks[li]) {
+            if (MARKS && marks[li]) {
                 unsigned mk = marks[li];
 #pragma unroll
                 for (int d = 0; d < 8; d++)
@@ -1028,9 +1030,9 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     int bpc = 0;
     hipError_t oe = hipSuccess;
     switch (c->plat_k) {
-    case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<4, TS>, PLAT_NT, 0); break;
-    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<16, TS>, PLAT_NT, 0); break;
-    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<8, TS>, PLAT_NT, 0); break;
+    case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<4, TS, false>, PLAT_NT, 0); break;
+    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<16, TS, false>, PLAT_NT, 0); break;
+    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_plat_block<8, TS, false>, PLAT_NT, 0); break;
     }
     if (oe != hipSuccess || bpc < 1) bpc = 1;
     int G = bpc * c->ncu;
@@ -1044,8 +1046,12 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
             int *a_next = L[(launch + 1) % 3];
             switch (c->plat_k) {
 #define PLAT_LAUNCH(K)                                                                                           \
-    k_plat_block<K, TS><<<G, PLAT_NT, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, tver, eqb, c->fd,   \
-                                                 c->SFD, slopes, c->NP, tile_has, tstamp, ctl)
+    if (slopes)                                                                                                  \
+        k_plat_block<K, TS, true><<<G, PLAT_NT, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, tver, eqb, \
+                                                           c->fd, c->SFD, slopes, c->NP, tile_has, tstamp, ctl);      \
+    else                                                                                                         \
+        k_plat_block<K, TS, false><<<G, PLAT_NT, 0, c->s>>>(W, H, launch, r, a_cur, a_next, sp_in, sp_out, tver, eqb, \
+                                                            c->fd, c->SFD, slopes, c->NP, tile_has, tstamp, ctl)
             case 4: PLAT_LAUNCH(4); break;
             case 16: PLAT_LAUNCH(16); break;
             default: PLAT_LAUNCH(8); break;
