@@ -64,6 +64,23 @@ __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
 template <int TW> __host__ __device__ constexpr int poff(int d) { return cdy(d) * (TW + 2) + cdx(d); }
 template <int TW> __device__ __forceinline__ int pidx(int lr, int lc) { return (lr + 1) * (TW + 2) + lc + 1; }
 
+// padded offset of direction (k + 2) & 7 for a bit index k = 0..7 (E, SE, S,
+// SW, W, NW, N, NE): bit k of a donor mask in pull order, and bit k of an
+// ESRI code, both name that direction.  Two 2-bit tables, no branches.
+__host__ __device__ constexpr uint32_t kpack(bool ydir)
+{
+    uint32_t r = 0;
+    for (int k = 0; k < 8; k++) r |= (uint32_t)((ydir ? cdy((k + 2) & 7) : cdx((k + 2) & 7)) + 1) << (2 * k);
+    return r;
+}
+template <int TW> __device__ __forceinline__ int koff(int k)
+{
+    constexpr uint32_t KX = kpack(false), KY = kpack(true);
+    const int dx = (int)((KX >> (2 * k)) & 3u) - 1;
+    const int dy = (int)((KY >> (2 * k)) & 3u) - 1;
+    return dy * (TW + 2) + dx;
+}
+
 // position (nr, nc) relative to the tile of ring cell h of a TW x TH tile: top
 // row, bottom row (both with corners), left column, right column
 template <int TW, int TH>
@@ -82,15 +99,17 @@ __device__ __forceinline__ void fa_clamp(float2 &a) { if (a.x < 0.0f) a = make_f
 __device__ __forceinline__ double fa_final(double w, double a) { double v = w; v += a; return v; }
 __device__ __forceinline__ float2 fa_final(float w, float2 a) { return ff_add(make_float2(w, 0.f), a); }
 
-// Cell body interface used by the round kernel (li: index in the tile, p:
-// padded index):
+// Cell body interface used by the round kernel (p: padded index; every LDS
+// array of a body is padded, so one index serves all of them):
 //   Reg  load(i)        the cell's own inputs (tile cells, loaded in phase 1)
 //   HReg load_halo(i)   what a receiver reads of a donor in another tile
-//   Fin  store(L, li, p, r, &extra)  put the walk-visible values into LDS and
+//   Fin  store(L, p, r, &extra)  put the walk-visible values into LDS and
 //        return what the cell writes to HBM when it completes that does not
 //        depend on its donors (kept in registers); extra: word flags
-//   compute(L, li, p, bits, upd)  the chain part, LDS only
-//   finish(L, li, p, i, fin, upd)  HBM writes of a cell completed in this
+//   compute(L, p, bits, upd)  the chain part, LDS only; the donors are the
+//        set bits of the pull-order mask, summed lowest bit first, which is
+//        the order E..NE of the reference (absent donors add nothing)
+//   finish(L, p, i, fin, upd)  HBM writes of a cell completed in this
 //        round, after the walks (coalesced, off the chains)
 template <typename T, int TW, int TH>
 struct FaCell {
@@ -103,7 +122,7 @@ struct FaCell {
     FS *fa;
     struct Lds {
         FS val[NP];
-        T w[NC];
+        T w[NP];
     };
     // loads are issued unconditionally and stored to LDS after one wait; the
     // value of a cell that is not final yet is never read
@@ -115,31 +134,28 @@ struct FaCell {
     struct Fin {};
     __device__ __forceinline__ Reg load(long i) const { return Reg{w[i], fa[i]}; }
     __device__ __forceinline__ HReg load_halo(long i) const { return fa[i]; }
-    __device__ __forceinline__ Fin store(Lds &L, int li, int p, const Reg &r, uint32_t *) const
+    __device__ __forceinline__ Fin store(Lds &L, int p, const Reg &r, uint32_t *) const
     {
-        L.w[li] = r.w;
+        L.w[p] = r.w;
         L.val[p] = r.v;
         return Fin{};
     }
     __device__ __forceinline__ void store_halo(Lds &L, int p, const HReg &r) const { L.val[p] = r; }
     // fa = w + max(0, sum of donor fa in the order E..NE) (mfd_simple.cu:71-168;
     // prop = 1 for every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
-    __device__ __forceinline__ void compute(Lds &L, int li, int p, unsigned bits, bool) const
+    __device__ __forceinline__ void compute(Lds &L, int p, unsigned bits, bool) const
     {
-        // every neighbour and the weight in flight at once (the padded layout
-        // keeps all eight reads inside the array), then the donors in order
-        FS v[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = L.val[p + poff<TW>(cpull(k))];
-        const T wv = L.w[li];
+        const T wv = L.w[p];
         FS accum{};
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (bits & (1u << k)) fa_acc(accum, v[k]);
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1;
+            fa_acc(accum, L.val[p + koff<TW>(k)]);
+        }
         fa_clamp(accum);
         L.val[p] = fa_final(wv, accum);
     }
-    __device__ __forceinline__ void finish(const Lds &L, int, int p, long i, const Fin &, bool) const { fa[i] = L.val[p]; }
+    __device__ __forceinline__ void finish(const Lds &L, int p, long i, const Fin &, bool) const { fa[i] = L.val[p]; }
 };
 
 // MFDsedbud body (mfd_accum.cu:63-252) split by what it reads:
@@ -163,7 +179,7 @@ struct SedCell {
     double *soilT;
     struct Lds {
         T stone[NP];
-        T a[NC], b[NC];
+        T a[NP], b[NP];
     };
     struct Reg {
         T e, in, g, s, dp;
@@ -176,7 +192,7 @@ struct SedCell {
     };
     __device__ __forceinline__ Reg load(long i) const { return Reg{ero[i], inci[i], geli[i], stone[i], dps[i], soilT[i]}; }
     __device__ __forceinline__ HReg load_halo(long i) const { return stone[i]; }
-    __device__ __forceinline__ Fin store(Lds &L, int li, int p, const Reg &r, uint32_t *extra) const
+    __device__ __forceinline__ Fin store(Lds &L, int p, const Reg &r, uint32_t *extra) const
     {
         L.stone[p] = r.s;
         double sT = r.sT;
@@ -189,8 +205,8 @@ struct SedCell {
             else { a = (T)old * r.s; b = (T)input; }
             *extra |= W_UPD;
         }
-        L.a[li] = a;
-        L.b[li] = b;
+        L.a[p] = a;
+        L.b[p] = b;
         double toterosion;
         if (sizeof(T) == 8) toterosion = (double)r.e + (double)r.in + (double)r.g;
         else toterosion = (double)(r.e + r.in + r.g);
@@ -200,24 +216,26 @@ struct SedCell {
         return f;
     }
     __device__ __forceinline__ void store_halo(Lds &L, int p, const HReg &r) const { L.stone[p] = r; }
-    __device__ __forceinline__ void compute(Lds &L, int li, int p, unsigned bits, bool upd) const
+    // the reference sums all eight terms with zeros for absent donors; adding
+    // only the donors' terms from +0 gives the same value up to the sign of a
+    // zero sum, which the clamp below erases
+    __device__ __forceinline__ void compute(Lds &L, int p, unsigned bits, bool upd) const
     {
         if (!upd) return;
         const T one = (T)1;
-        T nb[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) nb[k] = L.stone[p + poff<TW>(cpull(k))];
-        const T a = L.a[li], b = L.b[li];
-        T from[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) from[k] = (bits & (1u << k)) ? one * nb[k] : (T)0;
-        T propStonefrom = (from[0] + from[1] + from[2] + from[3] + from[4] + from[5] + from[6] + from[7]);
+        const T a = L.a[p], b = L.b[p];
+        T propStonefrom = (T)0;
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1;
+            propStonefrom = propStonefrom + one * L.stone[p + koff<TW>(k)];
+        }
         T st = a + b * propStonefrom;
         if (st < (T)0.01) st = (T)0.01;
         else if (st > (T)99.9) st = (T)99.9;
         L.stone[p] = st;
     }
-    __device__ __forceinline__ void finish(const Lds &L, int, int p, long i, const Fin &f, bool upd) const
+    __device__ __forceinline__ void finish(const Lds &L, int p, long i, const Fin &f, bool upd) const
     {
         depo[i] = f.dp;
         if (upd) {
@@ -367,6 +385,11 @@ struct DagCtl {
 // HBM copies are read by other tiles in later launches.  So ordering a
 // resolved cell's LDS writes before the counter update that can hand it to
 // another thread needs only the LDS queue drained, not the global stores.
+// threads of a workgroup that walk chains (whole waves)
+#ifndef DAG_WALKERS
+#define DAG_WALKERS 64
+#endif
+
 __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // One round over the woken tiles.  Inside a tile every cell gets a counter of
@@ -391,8 +414,13 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
     constexpr int PER = NC / 256;
     constexpr int NH = 2 * (TW + 2) + 2 * TH;
     constexpr int HPER = (NH + 255) / 256;
+    constexpr int NP = (TW + 2) * (TH + 2);
     static_assert(NC % 256 == 0, "tile shape");
-    __shared__ uint32_t wd[NC];
+    // packed words (W_*) in the padded layout; a ring cell's word has state 3,
+    // a counter no decrement brings to zero, and in bits 8-16 the one-hot 3x3
+    // position of the neighbour tile it belongs to, so a hand-over to it
+    // marks that tile for waking without a bounds test
+    __shared__ uint32_t wd[NP];
     // states at the start of the round, padded layout: tile cells ST_PENDING,
     // ST_DONE, ST_DEAD; ring cells the same, with a donor in another tile
     // that is not final from an earlier launch as ST_RING_WAIT
@@ -400,6 +428,7 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
     __shared__ typename Cell::Lds lds;
     __shared__ uint16_t spill[2][NC];
     __shared__ int nspill[2];
+    __shared__ int take;
     __shared__ int any_pending;
     __shared__ unsigned exits;
     const int n = ctl->cnt[round % 3];
@@ -410,7 +439,7 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         if (tile_done[tile]) continue;
         const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
         __syncthreads();
-        if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; exits = 0; }
+        if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; take = 0; exits = 0; }
         // 1. load states and cached values of the tile and of its ring (all
         //    loads in flight before the first wait); a ring cell counts as
         //    final only if it completed in an earlier launch
@@ -460,8 +489,8 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
             uint32_t sv = (v == DAG_PENDING) ? ST_PENDING : (v >= 0 ? ST_DONE : ST_DEAD);
             xw[q] = 0;
             const int p = pidx<TW>(li / TW, li % TW);
-            fin[q] = cell.store(lds, li, p, vr[q], &xw[q]);
-            wd[li] = (sv << W_STSH) | ((uint32_t)vf[q] << 16) | ((uint32_t)vb[q] << 8) | W_NEVER;
+            fin[q] = cell.store(lds, p, vr[q], &xw[q]);
+            wd[p] = (sv << W_STSH) | ((uint32_t)vf[q] << 16) | ((uint32_t)vb[q] << 8) | W_NEVER;
             pst[p] = (uint8_t)sv;
         }
 #pragma unroll
@@ -478,6 +507,8 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                 const int p = pidx<TW>(nr, nc);
                 pst[p] = hs;
                 cell.store_halo(lds, p, hr[q]);
+                const int ty = nr < 0 ? 0 : (nr >= TH ? 2 : 1), tx = nc < 0 ? 0 : (nc >= TW ? 2 : 1);
+                wd[p] = (3u << W_STSH) | (1u << (8 + ty * 3 + tx)) | W_NEVER;
             }
         }
         __syncthreads();
@@ -511,34 +542,43 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < PER; q++)
-            if ((pend0 >> q) & 1) wd[threadIdx.x + q * 256] = myw[q];
+        for (int q = 0; q < PER; q++) {
+            const int li = threadIdx.x + q * 256;
+            const int p = pidx<TW>(li / TW, li % TW);
+            if ((pend0 >> q) & 1) wd[p] = myw[q];
+            // the seeds form the work list of the first pass (list 1)
+            const bool sd = (seeds >> q) & 1;
+            const unsigned long long m = __ballot(sd);
+            int base = 0;
+            if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(&nspill[1], __popcll(m));
+            base = __shfl(base, 0);
+            if (sd) spill[1][base + __popcll(m & ((1ULL << (threadIdx.x & 63)) - 1))] = (uint16_t)p;
+        }
         __syncthreads();
-        // 3. walks
+        // 3. walks.  Pass p works through list (p - 1) & 1 and collects the
+        //    extra ready receivers (MFD) in list p & 1.  Only the first
+        //    DAG_WALKERS threads walk: each takes the next list entry whenever
+        //    its chain ends, so its lanes stay busy while the list holds work,
+        //    and the other waves leave the SIMDs to other workgroups.
         unsigned my_exits = 0;
         for (int pass = 0;; pass++) {
-            int nwork = (pass == 0) ? 0 : nspill[(pass - 1) & 1];
-            if (pass > 0 && nwork == 0) break;
-            for (int w0 = 0;; w0++) {
+            const int nwork = nspill[(pass - 1) & 1];
+            if (nwork == 0) break;
+            if (threadIdx.x < DAG_WALKERS) {
+                const uint16_t *lst = spill[(pass - 1) & 1];
                 int cur = -1;
                 uint32_t word = 0;
-                if (pass == 0) {
-                    if (w0 >= PER) break;
-                    if (!((seeds >> w0) & 1)) continue;
-                    cur = threadIdx.x + w0 * 256;
-                    word = wd[cur];   // only dead-count and counter bits of a dead seed can have moved
-                } else {
-                    int idx = threadIdx.x + w0 * 256;
-                    if (idx >= nwork) break;
-                    cur = spill[(pass - 1) & 1][idx];
-                    word = wd[cur];
-                }
-                while (cur >= 0) {
-                    const int lr = cur / TW, lc = cur % TW;
+                for (;;) {
+                    if (cur < 0) {
+                        const int idx = atomicAdd(&take, 1);
+                        if (idx >= nwork) break;
+                        cur = lst[idx];
+                        word = wd[cur];   // only dead-count and counter bits of a dead seed can have moved
+                    }
                     const bool dead = Cell::kCanDie && ((word >> 24) & 0xFu) != 0;
                     uint32_t ns = ST_DEAD;
                     if (!dead) {
-                        cell.compute(lds, cur, pidx<TW>(lr, lc), (word >> 8) & 0xFFu, (word & W_UPD) != 0);
+                        cell.compute(lds, cur, (word >> 8) & 0xFFu, (word & W_UPD) != 0);
                         ns = ST_DONE;
                     }
                     wd[cur] = (word & ~(3u << W_STSH)) | (ns << W_STSH);
@@ -549,22 +589,17 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                     unsigned f = (word >> 16) & 0xFFu;
                     int nxt = -1;
                     uint32_t nword = 0;
-                    // receivers: the set bits of the fd code (bit b is direction (b + 2) & 7)
+                    // receivers: the set bits of the fd code (bit k is direction (k + 2) & 7)
                     while (f) {
-                        const int d = (__ffs(f) + 1) & 7;
+                        const int k = __ffs(f) - 1;
                         f &= f - 1;
-                        int nr = lr + cdy(d), nc = lc + cdx(d);
-                        if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) {
-                            int rl = nr * TW + nc;
-                            uint32_t old = atomicAdd(&wd[rl], delta);
-                            if ((old & W_CNT) == 1u) {
-                                if (nxt < 0) { nxt = rl; nword = old + delta; }
-                                else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = (uint16_t)rl;
-                            }
-                        } else {
-                            // receiver in a neighbour tile: 3x3 position of that tile
-                            int ty = nr < 0 ? 0 : (nr >= TH ? 2 : 1), tx = nc < 0 ? 0 : (nc >= TW ? 2 : 1);
-                            my_exits |= 1u << (ty * 3 + tx);
+                        const int rp = cur + koff<TW>(k);
+                        const uint32_t old = atomicAdd(&wd[rp], delta);
+                        // a ring word carries the neighbour tile's position
+                        my_exits |= (((old >> W_STSH) & 3u) == 3u) ? ((old >> 8) & 0x1FFu) : 0u;
+                        if ((old & W_CNT) == 1u) {
+                            if (nxt < 0) { nxt = rp; nword = old + delta; }
+                            else spill[pass & 1][atomicAdd(&nspill[pass & 1], 1)] = (uint16_t)rp;
                         }
                     }
                     cur = nxt;
@@ -572,7 +607,7 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) nspill[(pass + 1) & 1] = 0;
+            if (threadIdx.x == 0) { nspill[(pass + 1) & 1] = 0; take = 0; }
             __syncthreads();
         }
         if (my_exits) atomicOr(&exits, my_exits);
@@ -582,12 +617,13 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         for (int q = 0; q < PER; q++) {
             if (!((pend0 >> q) & 1)) continue;
             int li = threadIdx.x + q * 256;
-            uint32_t wv = wd[li];
+            const int p = pidx<TW>(li / TW, li % TW);
+            uint32_t wv = wd[p];
             uint32_t s2 = (wv >> W_STSH) & 3u;
             if (s2 == ST_PENDING) { pend = true; continue; }
             long i = (long)(ty0 + li / TW) * W + (tx0 + li % TW);
             if (s2 == ST_DONE) {
-                cell.finish(lds, li, pidx<TW>(li / TW, li % TW), i, fin[q], (wv & W_UPD) != 0);
+                cell.finish(lds, p, i, fin[q], (wv & W_UPD) != 0);
                 st[i] = round;
             } else {
                 st[i] = -(round + 1);
