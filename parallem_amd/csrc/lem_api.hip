@@ -183,7 +183,7 @@ static void free_all(lemhip_ctx *c)
                     c->inci, c->geli, c->depo, c->dz, c->wC, c->wP, c->sp0, c->sp1, c->cand, c->ws, c->scan_tmp,
                     c->z2, c->pl_tiles, c->pl_ctl, c->cub_tmp,
                     c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
-                    c->dmask, c->tile_flags, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
+                    c->dmask, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
                     c->dag_ctl, c->partials, c->scal, c->demid, c->ddesc, c->dsc, c->slot_flags, c->labdem,
                     c->dstage};
     for (void *q : ptrs)
@@ -281,7 +281,6 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
         if (nt > ntiles) ntiles = nt;
     }
     c->dag_tiles_max = ntiles;
-    LEM_TRY(A((void **)&c->tile_flags, ntiles * 4));
     for (int q = 0; q < 3; q++) LEM_TRY(A((void **)&c->dag_list[q], ntiles * 4));
     LEM_TRY(A((void **)&c->dag_wake, ntiles * 4));
     LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));

@@ -112,9 +112,8 @@ struct lemhip_ctx {
     long lcap = 0;
     uint8_t *status = nullptr;  // DAG engine status
     uint8_t *dmask = nullptr;   // DAG donor bits (pull order)
-    int *tile_flags = nullptr;  // per tile: 1 = finished
     int *dag_list[3] = {nullptr, nullptr, nullptr};   // rotating lists of woken tiles
-    int *dag_wake = nullptr;                          // per tile: last round it was queued for
+    int *dag_wake = nullptr;                          // per tile: last round it was queued for, DAG_TILE_DONE when finished
     void *dag_ctl = nullptr;
     int dag_round = 0;
     int dag_rounds_last = 0;

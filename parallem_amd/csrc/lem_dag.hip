@@ -14,6 +14,7 @@
 #include <string.h>
 
 #define DAG_PENDING 0x7FFFFFFF
+#define DAG_TILE_DONE 0x7FFFFFFF   // wake stamp of a tile with no pending cell
 // status word: DAG_PENDING, r >= 0 done in round r (0 = before the first
 // round), -(r+1) dead (can never complete) since round r
 
@@ -407,7 +408,7 @@ __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)
 template <int TW, int TH, class Cell>
 __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
                                                    const uint8_t *__restrict__ fdv, int *__restrict__ st,
-                                                   int *__restrict__ tile_done, const int *__restrict__ act,
+                                                   const int *__restrict__ act,
                                                    int *__restrict__ next, int *__restrict__ wake, DagCtl *ctl)
 {
     constexpr int NC = TW * TH;
@@ -431,12 +432,15 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
     __shared__ int take;
     __shared__ int any_pending;
     __shared__ unsigned exits;
+    // the list count and this workgroup's first entry are read together (the
+    // grid never exceeds the list's capacity; an entry past the count is
+    // never used)
     const int n = ctl->cnt[round % 3];
+    const int first = act[blockIdx.x];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(round + 2) % 3] = 0;
     const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH;
     for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
-        const int tile = act[q0];
-        if (tile_done[tile]) continue;
+        const int tile = (q0 == (int)blockIdx.x) ? first : act[q0];
         const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
         __syncthreads();
         if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; take = 0; exits = 0; }
@@ -639,7 +643,8 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                 int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
                 if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
                     t2 = ty * ntx + tx;
-                    if (atomicExch(&wake[t2], round + 1) == round + 1) t2 = -1;
+                    // queued already, or finished
+                    if (atomicMax(&wake[t2], round + 1) >= round + 1) t2 = -1;
                 }
             }
             unsigned long long m = __ballot(t2 >= 0);
@@ -650,7 +655,7 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         }
         if (threadIdx.x == 0) {
             if (prog) ctl->flag = 1;
-            if (!any_pending) tile_done[tile] = 1;
+            if (!any_pending) atomicExch(&wake[tile], DAG_TILE_DONE);
         }
     }
 }
@@ -688,15 +693,13 @@ template <int TW, int TH, class Cell>
 static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
 {
     int ntiles = ((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
-    LEM_CHECK(c, hipMemsetAsync(c->tile_flags, 0, sizeof(int) * ntiles, c->s));
     int *st = reinterpret_cast<int *>(c->scan_tmp);
     int *L[3] = {c->dag_list[0], c->dag_list[1], c->dag_list[2]};
     DagCtl *ctl = reinterpret_cast<DagCtl *>(c->dag_ctl);
     int r = c->dag_round;
-    if (r > (1 << 29)) {  // wake stamps and status words hold round numbers
-        LEM_CHECK(c, hipMemsetAsync(c->dag_wake, 0, sizeof(int) * c->dag_tiles_max, c->s));
-        r = 0;
-    }
+    if (r > (1 << 29)) r = 0;   // status words hold round numbers
+    // wake stamps start below every round number of the pass
+    LEM_CHECK(c, hipMemsetAsync(c->dag_wake, 0, sizeof(int) * ntiles, c->s));
     // the first round visits every tile
     k_iota<<<(ntiles + 255) / 256, 256, 0, c->s>>>(ntiles, L[(r + 1) % 3]);
     LEM_LAUNCHED(c);
@@ -716,8 +719,8 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
         prof_begin(c, prof_slot);
         for (int q = 0; q < BATCH; q++) {
             ++r;
-            k_dag_round<TW, TH, Cell><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, c->tile_flags,
-                                                           L[r % 3], L[(r + 1) % 3], c->dag_wake, ctl);
+            k_dag_round<TW, TH, Cell><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, L[r % 3],
+                                                           L[(r + 1) % 3], c->dag_wake, ctl);
         }
         prof_end(c, prof_slot, BATCH);
         LEM_LAUNCHED(c);
