@@ -529,16 +529,20 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
             pend0 |= 1 << q;
             const int p = pidx<TW>(li / TW, li % TW);
             unsigned bits = vb[q];
+            // all eight states in flight at once (the padded layout keeps
+            // them inside the array), then a one-hot code per donor
+            uint32_t s8[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) s8[k] = pst[p + poff<TW>(cpull(k))];
             int nin = 0;
-            bool dead = false, parked = false;
+            uint32_t seen = 0;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                if (!(bits & (1u << k))) continue;
-                uint32_t s2 = pst[p + poff<TW>(cpull(k))];
-                nin += (s2 == ST_PENDING);
-                parked |= (s2 == ST_RING_WAIT);
-                dead |= (s2 == ST_DEAD);
+                const uint32_t oh = ((bits >> k) & 1u) << s8[k];
+                nin += (int)(oh & (1u << ST_PENDING));
+                seen |= oh;
             }
+            const bool dead = (seen >> ST_DEAD) & 1u, parked = (seen >> ST_RING_WAIT) & 1u;
             uint32_t base = ((uint32_t)ST_PENDING << W_STSH) | ((uint32_t)vf[q] << 16) | (bits << 8) | xw[q];
             if (dead) { myw[q] = base | W_NEVER | W_DEAD1; seeds |= 1 << q; }
             else if (!parked) { myw[q] = base | (uint32_t)nin; if (nin == 0) seeds |= 1 << q; }
