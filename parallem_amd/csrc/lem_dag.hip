@@ -59,7 +59,7 @@ __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
 // cell once all donors are final.  Walks then never wait on global loads.
 // Values that receivers read are kept in a padded (TW+2) x (TH+2) layout that
 // holds the ring too, so a donor is one LDS read at a fixed offset from the
-// cell's padded index; values only the cell itself reads are unpadded.
+// cell's padded index; the cell's own values use the same layout.
 
 // padded index of neighbour d of the cell at padded index p
 template <int TW> __host__ __device__ constexpr int poff(int d) { return cdy(d) * (TW + 2) + cdx(d); }
@@ -146,6 +146,8 @@ struct FaCell {
     // prop = 1 for every mask donor, runoffweight.cu:126-134, so fa*prop == fa)
     __device__ __forceinline__ void compute(Lds &L, int p, unsigned bits, bool) const
     {
+        // one read per donor (chain cells mostly have one or two donors, and
+        // a compensated add costs more than the read)
         const T wv = L.w[p];
         FS accum{};
         while (bits) {
@@ -225,12 +227,15 @@ struct SedCell {
         if (!upd) return;
         const T one = (T)1;
         const T a = L.a[p], b = L.b[p];
+        // all eight neighbours in flight at once (the padded layout keeps the
+        // reads inside the array), then the donors' terms in order
+        T v8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v8[k] = L.stone[p + poff<TW>(cpull(k))];
         T propStonefrom = (T)0;
-        while (bits) {
-            const int k = __ffs(bits) - 1;
-            bits &= bits - 1;
-            propStonefrom = propStonefrom + one * L.stone[p + koff<TW>(k)];
-        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (bits & (1u << k)) propStonefrom = propStonefrom + one * v8[k];
         T st = a + b * propStonefrom;
         if (st < (T)0.01) st = (T)0.01;
         else if (st > (T)99.9) st = (T)99.9;
@@ -382,14 +387,15 @@ struct DagCtl {
 };
 
 
-// Inside a tile, donors are read from LDS only (values, states, counters);
-// HBM copies are read by other tiles in later launches.  So ordering a
-// resolved cell's LDS writes before the counter update that can hand it to
-// another thread needs only the LDS queue drained, not the global stores.
 // threads of a workgroup that walk chains (whole waves)
 #ifndef DAG_WALKERS
 #define DAG_WALKERS 64
 #endif
+
+// Inside a tile, donors are read from LDS only (values, states, counters);
+// HBM copies are read by other tiles in later launches.  So ordering a
+// resolved cell's LDS writes before the counter update that can hand it to
+// another thread needs only the LDS queue drained, not the global stores.
 
 __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
