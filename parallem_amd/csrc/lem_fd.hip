@@ -677,12 +677,15 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 // edge list keeps the minimum per unordered pair (order-free, exact).  The
 // cells of a block share most of their basin pairs (a boundary runs through
 // many cells): pairs are first merged in an LDS table (minimum weight per
-// pair), and each block appends its distinct pairs to a list with one atomic;
-// pairs that find no LDS slot are appended one by one.  The list is then
+// pair), and each block appends its distinct pairs to a list with one atomic
+// (blocks of 64 x 32 cells: appends from many blocks to one counter
+// serialise, so a block covers many cells); pairs that find no LDS slot are
+// appended one by one.  The list is then
 // sorted by pair and reduced to the minimum weight per pair.  Keys pack
 // (lo << B) | hi with B bits per label.  The count may exceed the capacity:
 // the caller then grows the list and runs again.
-#define EDGE_LDS 1024
+#define EDGE_LDS 2048
+#define EDGE_RG 8   // row groups of 4 per block: 64 x (4 * EDGE_RG) cells
 __global__ void __launch_bounds__(256) k_edges_emit(int W, int H, const uint8_t *__restrict__ mask,
                                                     const int *__restrict__ ws, const double *__restrict__ z,
                                                     const uint8_t *__restrict__ demid, unsigned long long *keys,
@@ -695,39 +698,55 @@ __global__ void __launch_bounds__(256) k_edges_emit(int W, int H, const uint8_t 
     for (int q = tid; q < EDGE_LDS; q += 256) { lk[q] = HASH_EMPTY; lv[q] = HASH_EMPTY; }
     __syncthreads();
     const int icol = blockIdx.x * blockDim.x + threadIdx.x;
-    const int irow = blockIdx.y * blockDim.y + threadIdx.y;
-    const long c = (long)irow * W + icol;
-    if (icol < W && irow < H && mask[c]) {
-        int me = ws[c];
-        double zc = z[c];
-        unsigned long long wenc = ord_enc(zc);
-        int seen[8];
-        int ns = 0;
-        for (int d = 0; d < 8; d++) {
-            int cx = icol + kDX[d], cy = irow + kDY[d];
-            if (cx < 0 || cx >= W || cy < 0 || cy >= H) continue;
-            long nb = (long)cy * W + cx;
-            // a neighbour outside the cell's own DEM is off-grid for that DEM
-            if (demid && demid[nb] != demid[c]) continue;
-            int other = ws[nb];
-            if (other == me) continue;
-            if (!(zc >= z[nb])) continue;
-            bool dup = false;
-            for (int q = 0; q < ns; q++) dup |= (seen[q] == other);
-            if (dup) continue;
-            seen[ns++] = other;
-            unsigned long long lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
-            unsigned long long key = (lo << B) | hi;
-            unsigned h = (unsigned)(mix64(key) & (EDGE_LDS - 1));
-            bool done = false;
-            for (int p = 0; p < 16 && !done; p++) {
-                unsigned long long prev = atomicCAS(&lk[h], HASH_EMPTY, key);
-                if (prev == HASH_EMPTY || prev == key) { atomicMin(&lv[h], wenc); done = true; }
-                h = (h + 1) & (EDGE_LDS - 1);
+    for (int g = 0; g < EDGE_RG; g++) {
+        const int irow = (blockIdx.y * EDGE_RG + g) * blockDim.y + threadIdx.y;
+        const long c = (long)irow * W + icol;
+        if (icol < W && irow < H && mask[c]) {
+            int me = ws[c];
+            double zc = z[c];
+            unsigned long long wenc = ord_enc(zc);
+            // the eight neighbours' labels, heights and DEM ids in flight at
+            // once (a wait per neighbour inside the branches below would
+            // serialise them); an off-grid neighbour reads the cell itself,
+            // whose label never makes an edge
+            int oth[8];
+            double zn[8];
+            uint8_t dn[8];
+#pragma unroll
+            for (int d = 0; d < 8; d++) {
+                const int cx = icol + cdx(d), cy = irow + cdy(d);
+                const long nb = (cx >= 0 && cx < W && cy >= 0 && cy < H) ? (long)cy * W + cx : c;
+                oth[d] = ws[nb];
+                zn[d] = z[nb];
+                dn[d] = demid ? demid[nb] : 0;
             }
-            if (!done) {
-                int at = atomicAdd(count, 1);
-                if (at < cap) { keys[at] = key; vals[at] = wenc; }
+            const int mydem = demid ? demid[c] : 0;
+            int seen[8];
+            int ns = 0;
+#pragma unroll
+            for (int d = 0; d < 8; d++) {
+                const int other = oth[d];
+                if (other == me) continue;
+                // a neighbour outside the cell's own DEM is off-grid for that DEM
+                if (dn[d] != mydem) continue;
+                if (!(zc >= zn[d])) continue;
+                bool dup = false;
+                for (int q = 0; q < ns; q++) dup |= (seen[q] == other);
+                if (dup) continue;
+                seen[ns++] = other;
+                unsigned long long lo = (unsigned)(me < other ? me : other), hi = (unsigned)(me < other ? other : me);
+                unsigned long long key = (lo << B) | hi;
+                unsigned h = (unsigned)(mix64(key) & (EDGE_LDS - 1));
+                bool done = false;
+                for (int p = 0; p < 16 && !done; p++) {
+                    unsigned long long prev = atomicCAS(&lk[h], HASH_EMPTY, key);
+                    if (prev == HASH_EMPTY || prev == key) { atomicMin(&lv[h], wenc); done = true; }
+                    h = (h + 1) & (EDGE_LDS - 1);
+                }
+                if (!done) {
+                    int at = atomicAdd(count, 1);
+                    if (at < cap) { keys[at] = key; vals[at] = wenc; }
+                }
             }
         }
     }
@@ -1239,7 +1258,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     for (int attempt = 0;; ++attempt) {
         LEM_TRY(ensure_hash(c, want));
         LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
-        dim3 blk(64, 4), grd((W + 63) / 64, (H + 3) / 4);
+        dim3 blk(64, 4), grd((W + 63) / 64, (H + 4 * EDGE_RG - 1) / (4 * EDGE_RG));
         k_edges_emit<<<grd, blk, 0, c->s>>>(W, H, c->mask, c->ws, c->dem, c->demid, c->hkeys, c->hvals, c->hcap,
                                             &c->scal->nedges, B);
         LEM_LAUNCHED(c);
