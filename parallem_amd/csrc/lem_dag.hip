@@ -176,7 +176,7 @@ struct SedCell {
     static constexpr int NC = TW * TH;
     static constexpr int NP = (TW + 2) * (TH + 2);
     static constexpr bool kCanDie = true;
-    static constexpr int kWaves = 5;
+    static constexpr int kWaves = 4;   // fewer waves, but no spills in the FP32 form
     const T *ero, *inci, *geli, *dps;
     T *depo, *stone, *fines;
     double *soilT;
