@@ -540,9 +540,8 @@ __global__ void __launch_bounds__(256) k_count_sinks(const int *cand, int n, con
 
 // DEM-local border of init_watershedMFD (newflow.cu:371): last column, first
 // and last row of the cell's own DEM
-__device__ __forceinline__ bool dem_border(int W, int H, long i, const uint8_t *demid, const DemDesc *dd)
+__device__ __forceinline__ bool dem_border(int W, int H, int r, int c, long i, const uint8_t *demid, const DemDesc *dd)
 {
-    int r = (int)(i / W), c = (int)(i % W);
     if (demid) {
         DemDesc D = dd[demid[i]];
         r -= D.y0; c -= D.x0;
@@ -552,24 +551,29 @@ __device__ __forceinline__ bool dem_border(int W, int H, long i, const uint8_t *
 }
 
 // sink flag: mask, not on the DEM border, SFD == 0 (newflow.cu:363-378)
+// Row/column grids (grid2d): cell coordinates without a division per cell.
 __global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *SFD, const uint8_t *demid,
                              const DemDesc *dd, int *flag)
 {
-    long N = (long)W * H;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
-        flag[i] = (mask[i] && SFD[i] == 0 && !dem_border(W, H, i, demid, dd)) ? 1 : 0;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= W) return;
+    for (int r = blockIdx.y; r < H; r += gridDim.y) {
+        const long i = (long)r * W + c;
+        flag[i] = (mask[i] && SFD[i] == 0 && !dem_border(W, H, r, c, i, demid, dd)) ? 1 : 0;
+    }
 }
 
 // init_watershedMFD with labels = 1 + rank of the sink cell in atlas row-major order (D3)
 __global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD, const int *prefix,
                           const uint8_t *demid, const DemDesc *dd, int *ws, uint8_t *labdem)
 {
-    long N = (long)W * H;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        int r = (int)(i / W), c = (int)(i % W);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= W) return;
+    for (int r = blockIdx.y; r < H; r += gridDim.y) {
+        const long i = (long)r * W + c;
         int v;
         if (!mask[i]) v = 0;
-        else if (dem_border(W, H, i, demid, dd)) v = 0;
+        else if (dem_border(W, H, r, c, i, demid, dd)) v = 0;
         else if (SFD[i] == 0) {
             v = prefix[i] + 1;
             labdem[v] = (uint8_t)slot_of(demid, i);
@@ -984,6 +988,13 @@ static int grid1d(long n, int block = 256, int cap = 8192)
     return (int)g;
 }
 
+// 256-thread blocks over the columns, a row per block row (rows beyond the
+// grid's height are taken in strides)
+static dim3 grid2d(int W, int H)
+{
+    return dim3((unsigned)((W + 255) / 256), (unsigned)(H < 32768 ? (H > 0 ? H : 1) : 32768));
+}
+
 static int read_scalars(lemhip_ctx *c)
 {
     LEM_CHECK(c, hipMemcpyAsync(c->hscal, c->scal, sizeof(LemScalars), hipMemcpyDeviceToHost, c->s));
@@ -1179,7 +1190,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     long N = c->N;
     int W = c->W, H = c->H;
     // labels
-    k_sink_flags<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, c->demid, c->ddesc, c->scan_tmp);
+    k_sink_flags<<<grid2d(W, H), 256, 0, c->s>>>(W, H, c->mask, c->SFD, c->demid, c->ddesc, c->scan_tmp);
     LEM_LAUNCHED(c);
     size_t need = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->scan_tmp, c->ws, (int)N, c->s);
@@ -1194,7 +1205,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     LEM_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, need, c->scan_tmp, prefix, (int)N, c->s));
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[0], c->scan_tmp + (N - 1), sizeof(int), hipMemcpyDeviceToHost, c->s));
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[1], prefix + (N - 1), sizeof(int), hipMemcpyDeviceToHost, c->s));
-    k_init_ws<<<grid1d(N), 256, 0, c->s>>>(W, H, c->mask, c->SFD, prefix, c->demid, c->ddesc, c->ws, c->labdem);
+    k_init_ws<<<grid2d(W, H), 256, 0, c->s>>>(W, H, c->mask, c->SFD, prefix, c->demid, c->ddesc, c->ws, c->labdem);
     LEM_LAUNCHED(c);
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     int nlab = c->hint[0] + c->hint[1];
