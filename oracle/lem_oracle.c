@@ -289,7 +289,7 @@ int orc_watershed_labels(int W, int H, const double *zs, const int *mask, const 
 {
     long N = (long)W * H;
     int counter = 0;
-    /* A7 */
+    /* sinks get fresh labels, other cells point at their SFD receiver */
     for (int irow = 0; irow < H; irow++)
         for (int icol = 0; icol < W; icol++) {
             long self = (long)irow * W + icol;
@@ -302,7 +302,7 @@ int orc_watershed_labels(int W, int H, const double *zs, const int *mask, const 
                 ws[self] = -(celly * W + cellx + 1);
             }
         }
-    /* A8: every mask cell with SFD==0 adopts the largest label among its
+    /* every mask cell with SFD==0 adopts the largest label among its
      * equal-height neighbours until nothing changes.  The fixed point gives
      * each 8-connected equal-height component of SFD==0 mask cells its
      * largest label; union-find reaches the same fixed point directly. */
@@ -334,7 +334,7 @@ int orc_watershed_labels(int W, int H, const double *zs, const int *mask, const 
         if (mask[i] == 1 && SFD[i] == 0) ws[i] = cmax[uf_find(par, i)];
     free(cmax);
     free(par);
-    /* A9: pointer jumping to the terminal label (chains are acyclic: SFD
+    /* pointer jumping to the terminal label (chains are acyclic: SFD
      * pointers strictly decrease height or, on plateaus, distance) */
     long *stack = (long *)malloc(sizeof(long) * N);
     for (long i = 0; i < N; i++) {

@@ -55,7 +55,7 @@ void orc_SetClimateMatrices(const orc_params *p, const int *mask, const double *
                             double rain_scalar, double temp_scalar, double *rainmat, double *tempmat);
 void orc_setProcessMatrices(const orc_params *p, orc_state *s);
 
-/* ---- FD (A1-A13) ---- */
+/* ---- flow directions and routing ---- */
 void orc_flowdirs(int W, int H, int cell_size, const double *dem, const int *mask,
                   double *Slopes, int *SFD, int *mfd, int *sinkcounter, int *flatcounter);
 int  orc_route_plateaus(int W, int H, const double *dem, const int *mask,
@@ -67,7 +67,7 @@ int  orc_spill_heights(int W, int H, const double *dem, const int *mask, const i
                        int nlab, double *raise, int *sinkcount, int *nedges);
 void orc_flow_direction(const orc_params *p, orc_state *s, int iter);
 
-/* ---- FA (B1-B4) ---- */
+/* ---- runoff and flow accumulation ---- */
 void orc_runoff(const orc_params *p, orc_state *s);
 void orc_flow_accum(const orc_params *p, orc_state *s, int iter);
 int  orc_flow_accum_literal(int W, int H, const int *mask, const int *fd, const double *w, double *fa);

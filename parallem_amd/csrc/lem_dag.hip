@@ -257,7 +257,7 @@ struct SedCell {
 
 // Both prep kernels read each cell's 3x3 neighbourhood.  A 64 x 16 block of
 // cells stages its 66 x 18 region (rows coalesced) in LDS once; off-grid
-// positions are staged as nodata, which is how every rule treats them (D11).
+// positions are staged as nodata, which is how every rule treats them.
 #define PREP_RG 4   // rows per thread: the block covers 64 x (4 * PREP_RG) cells
 #define PREP_RW 66
 #define PREP_RH (4 * PREP_RG + 2)
@@ -302,8 +302,8 @@ __global__ void __launch_bounds__(256) k_fa_prep(int W, int H, const uint8_t *__
 
 // MFDsedbud never computes row 0 / column 0 (mfd_accum.cu:42) and nodata
 // cells are never ok (mfd_accum.cu:390-394); nodata carries fd = -9999, whose
-// low byte sets E, W, NW, N, NE, so a nodata (or, D11, off-grid) neighbour at
-// E, SE, S, SW or W is a donor that never completes (D9).  Also the
+// low byte sets E, W, NW, N, NE, so a nodata (or off-grid) neighbour at
+// E, SE, S, SW or W is a donor that never completes (sediment blocking).  Also the
 // deposition of every cell (mfd_accum.cu:63-200): the donors' erosion,
 // incision and gelifluction terms in the order E..NE plus the cell's own, each
 // weighted by its deposition fraction; it reads only inputs of the pass.
@@ -408,7 +408,7 @@ __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)
 // ready receivers (MFD) go to an LDS spill list processed after a barrier.
 // A chain leaving the tile only marks the neighbour tile in an LDS mask; the
 // tile wakes each marked neighbour once, after its walks.  A cell with a dead
-// donor dies (sediment blocking, D9).  Work of a cell that does not depend on
+// donor dies (sediment blocking).  Work of a cell that does not depend on
 // its donors' results (the cell body's pre-phase) is done for all cells of
 // the tile in parallel before the walks.
 template <int TW, int TH, class Cell>

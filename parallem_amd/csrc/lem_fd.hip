@@ -171,7 +171,7 @@ __device__ __forceinline__ T block_reduce_sum(T v, T *sh)
     return r;
 }
 
-// ------------------------------------------------------------- A5: lowest
+// ----------------------------------------------------------------- lowest
 
 // routeflow.cu:739-741: the outlet is lowered below the lowest cell after the
 // directions were computed on the unadjusted surface (one thread per slot)
@@ -200,11 +200,11 @@ __device__ __forceinline__ void slot_hist_flush(int *hist, DemScal *ds, int off,
         if (hist[j]) atomicAdd(reinterpret_cast<int *>(reinterpret_cast<char *>(&ds[j]) + off), hist[j]);
 }
 
-// ------------------------------------------------------------ A2: FlowDirs
+// ---------------------------------------------------------------- FlowDirs
 
 // newflow.cu:49-174.  One thread per cell; a wave covers 64 consecutive cells
 // of a row so the centre row and both neighbour rows are read coalesced.
-// Slopes are written only for downhill neighbours (sticky state, D7); pass 2
+// Slopes are written only for downhill neighbours (sticky: Slopes keep the first pass's values); pass 2
 // passes slopes == nullptr.  Cells without a downhill neighbour are appended
 // to the plateau candidate list (the cells whose initial distance is 2e7).
 // A 64 x 16 tile per 64 x 4 block: the tile's surface plus a one-cell ring is
@@ -290,9 +290,9 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
         if ((cand_rows >> rg) & 1) cand[base++] = (int)((long)(y0 + rg * 4 + threadIdx.y) * W + icol);
 }
 
-// ------------------------------------------------------ A3/A4: plateaus
+// ------------------------------------------------------------ plateaus
 //
-// route_plateausMFD (newflow.cu:264-350) in synchronous rounds (D1): in round
+// route_plateausMFD (newflow.cu:264-350) in synchronous rounds: in round
 // r a candidate reads its equal-height neighbours' round r-1 distances; each
 // strict improvement in the scan (order N..NW) marks that slope and takes its
 // direction.  The rounds are temporally blocked: one launch advances K rounds
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(256) k_count_sinks(const int *cand, int n, con
     slot_hist_flush(hist, ds, (int)offsetof(DemScal, sinks_pass1), ns);
 }
 
-// ------------------------------------------ A7-A9: watershed labels
+// ------------------------------------------------ watershed labels
 
 // DEM-local border of init_watershedMFD (newflow.cu:371): last column, first
 // and last row of the cell's own DEM
@@ -563,7 +563,7 @@ __global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *S
     }
 }
 
-// init_watershedMFD with labels = 1 + rank of the sink cell in atlas row-major order (D3)
+// init_watershedMFD with labels = 1 + rank of the sink cell in atlas row-major order
 __global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD, const int *prefix,
                           const uint8_t *demid, const DemDesc *dd, int *ws, uint8_t *labdem)
 {
@@ -665,7 +665,7 @@ __global__ void k_jump(long N, const uint8_t *mask, int *ws, int *flag)
     }
 }
 
-// ---------------------------------------------- A10-A11: edges, spill
+// ------------------------------------------------------ edges, spill
 
 #define HASH_EMPTY 0xFFFFFFFFFFFFFFFFULL
 
@@ -676,7 +676,7 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 }
 
 // identify_watershed_boundary_write_backMFD (newflow.cu:787-877) without the
-// fixed table (D4): every mask cell c and in-grid neighbour n with different
+// fixed-size table: every mask cell c and in-grid neighbour n with different
 // labels and z_c >= z_n contributes edge {ws_c, ws_n} at height z_c, and the
 // edge list keeps the minimum per unordered pair (order-free, exact).  The
 // cells of a block share most of their basin pairs (a boundary runs through
@@ -1395,7 +1395,7 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
     // lowest cell for the outlet check, taken on the unadjusted surface
     prof_begin(c, PROF_FD);
     LEM_TRY(lem_slot_lowest(c, c->dem, 0, c->nslot));
-    // pass 1 on the unraised surface; slopes persist (D7)
+    // pass 1 on the unraised surface; slopes keep these values
     int n1 = 0, r1 = 0;
     LEM_TRY(directions_and_plateaus<TS>(c, c->dem, reinterpret_cast<TS *>(c->slopes), &n1, &r1));
     k_slots_sinks_init<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->nslot, c->dsc);

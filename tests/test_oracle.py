@@ -323,7 +323,7 @@ def test_fa_conservation_and_outlet_cycle():
             term += fa[i]
     unc = o.s.fa_uncomputed
     cyc = [i for i in np.nonzero(mask == 1)[0] if fa[i] == w[i] and unc]
-    assert unc in (0, 2)   # the fd[outlet] = WEST rule can close a 2-cycle (D10)
+    assert unc in (0, 2)   # the fd[outlet] = WEST rule can close a 2-cycle (cells on an fd cycle are never computed)
     if unc == 0:
         assert math.isclose(term, w[mask == 1].sum(), rel_tol=1e-12)
 
@@ -383,7 +383,7 @@ def test_sediment_topological_equals_reference_round_replay(flowtype):
 
 
 def test_sediment_blocking_rule():
-    """A valid cell with nodata to its W, E, SE, S or SW never completes (D9);
+    """A valid cell with nodata to its W, E, SE, S or SW never completes (sediment blocking);
     nodata to its NW, N or NE does not block it; blocking passes downstream."""
     z = np.full((7, 7), -9999.0)
     z[1:6, 1:6] = np.linspace(50, 10, 25).reshape(5, 5)   # falls to the east and south
