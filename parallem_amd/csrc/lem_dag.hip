@@ -604,12 +604,24 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                     int nxt = -1;
                     uint32_t nword = 0;
                     // receivers: the set bits of the fd code (bit k is direction (k + 2) & 7)
-                    while (f) {
+                    if (f) {
+                        // the first (under single flow directions the only)
+                        // receiver, without branches: the walk continues with
+                        // it when the add hands it over
                         const int k = __ffs(f) - 1;
                         f &= f - 1;
                         const int rp = cur + koff<TW>(k);
                         const uint32_t old = atomicAdd(&wd[rp], delta);
                         // a ring word carries the neighbour tile's position
+                        my_exits |= (((old >> W_STSH) & 3u) == 3u) ? ((old >> 8) & 0x1FFu) : 0u;
+                        nxt = ((old & W_CNT) == 1u) ? rp : -1;
+                        nword = old + delta;
+                    }
+                    while (f) {   // further receivers (multiple flow directions)
+                        const int k = __ffs(f) - 1;
+                        f &= f - 1;
+                        const int rp = cur + koff<TW>(k);
+                        const uint32_t old = atomicAdd(&wd[rp], delta);
                         my_exits |= (((old >> W_STSH) & 3u) == 3u) ? ((old >> 8) & 0x1FFu) : 0u;
                         if ((old & W_CNT) == 1u) {
                             if (nxt < 0) { nxt = rp; nword = old + delta; }
