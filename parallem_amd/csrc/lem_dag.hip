@@ -74,12 +74,26 @@ __host__ __device__ constexpr uint32_t kpack(bool ydir)
     for (int k = 0; k < 8; k++) r |= (uint32_t)((ydir ? cdy((k + 2) & 7) : cdx((k + 2) & 7)) + 1) << (2 * k);
     return r;
 }
+// the eight offsets as signed bytes of one 64-bit constant (tiles up to 124
+// cells wide): one shift and one sign extension
+template <int TW> __host__ __device__ constexpr uint64_t kbytes()
+{
+    uint64_t r = 0;
+    for (int k = 0; k < 8; k++)
+        r |= (uint64_t)(uint8_t)(int8_t)(cdy((k + 2) & 7) * (TW + 2) + cdx((k + 2) & 7)) << (8 * k);
+    return r;
+}
 template <int TW> __device__ __forceinline__ int koff(int k)
 {
-    constexpr uint32_t KX = kpack(false), KY = kpack(true);
-    const int dx = (int)((KX >> (2 * k)) & 3u) - 1;
-    const int dy = (int)((KY >> (2 * k)) & 3u) - 1;
-    return dy * (TW + 2) + dx;
+    if constexpr (TW + 3 <= 127) {
+        constexpr uint64_t KB = kbytes<TW>();
+        return (int)(int8_t)(uint8_t)(KB >> (8 * k));
+    } else {
+        constexpr uint32_t KX = kpack(false), KY = kpack(true);
+        const int dx = (int)((KX >> (2 * k)) & 3u) - 1;
+        const int dy = (int)((KY >> (2 * k)) & 3u) - 1;
+        return dy * (TW + 2) + dx;
+    }
 }
 
 // position (nr, nc) relative to the tile of ring cell h of a TW x TH tile: top
@@ -411,7 +425,9 @@ __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)
 // donor dies (sediment blocking).  Work of a cell that does not depend on
 // its donors' results (the cell body's pre-phase) is done for all cells of
 // the tile in parallel before the walks.
-template <int TW, int TH, class Cell>
+// SFD: every cell has at most one receiver (single flow directions), so the
+// walk has no list of extra ready receivers
+template <int TW, int TH, class Cell, bool SFD>
 __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, int round, Cell cell, const uint8_t *__restrict__ dm,
                                                    const uint8_t *__restrict__ fdv, int *__restrict__ st,
                                                    const int *__restrict__ act,
@@ -617,7 +633,7 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                         nxt = ((old & W_CNT) == 1u) ? rp : -1;
                         nword = old + delta;
                     }
-                    while (f) {   // further receivers (multiple flow directions)
+                    while (!SFD && f) {   // further receivers (multiple flow directions)
                         const int k = __ffs(f) - 1;
                         f &= f - 1;
                         const int rp = cur + koff<TW>(k);
@@ -711,7 +727,7 @@ __global__ void k_iota(int n, int *a)
     if (i < n) a[i] = i;
 }
 
-template <int TW, int TH, class Cell>
+template <int TW, int TH, class Cell, bool SFD>
 static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
 {
     int ntiles = ((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
@@ -731,7 +747,7 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
     const int BATCH = c->dag_batch;
     // as many workgroups as the CUs hold at once (the first rounds visit every tile)
     int bpc = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_dag_round<TW, TH, Cell>, 256, 0) != hipSuccess || bpc < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_dag_round<TW, TH, Cell, SFD>, 256, 0) != hipSuccess || bpc < 1)
         bpc = 4;
     int G = bpc * c->ncu;
     if (G > ntiles) G = ntiles;
@@ -741,7 +757,7 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
         prof_begin(c, prof_slot);
         for (int q = 0; q < BATCH; q++) {
             ++r;
-            k_dag_round<TW, TH, Cell><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, L[r % 3],
+            k_dag_round<TW, TH, Cell, SFD><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, L[r % 3],
                                                            L[(r + 1) % 3], c->dag_wake, ctl);
         }
         prof_end(c, prof_slot, BATCH);
@@ -772,7 +788,8 @@ template <typename T, int TW, int TH>
 static int fa_shape(lemhip_ctx *c)
 {
     FaCell<T, TW, TH> cell{reinterpret_cast<const T *>(c->w), reinterpret_cast<typename FaStore<T>::type *>(c->fa)};
-    return run_rounds<TW, TH>(c, cell, &c->h_fa_unc, PROF_FA_K);
+    if (c->p.flowdirectiontype == 0) return run_rounds<TW, TH, decltype(cell), true>(c, cell, &c->h_fa_unc, PROF_FA_K);
+    return run_rounds<TW, TH, decltype(cell), false>(c, cell, &c->h_fa_unc, PROF_FA_K);
 }
 
 template <typename T, int TW, int TH>
@@ -782,7 +799,8 @@ static int sed_shape(lemhip_ctx *c)
                             reinterpret_cast<const T *>(c->geli), reinterpret_cast<const T *>(c->z2),
                             reinterpret_cast<T *>(c->depo), reinterpret_cast<T *>(c->stone),
                             reinterpret_cast<T *>(c->fines), c->soilT};
-    return run_rounds<TW, TH>(c, cell, &c->h_sed_unc, PROF_SED_K);
+    if (c->p.flowdirectiontype == 0) return run_rounds<TW, TH, decltype(cell), true>(c, cell, &c->h_sed_unc, PROF_SED_K);
+    return run_rounds<TW, TH, decltype(cell), false>(c, cell, &c->h_sed_unc, PROF_SED_K);
 }
 
 template <typename T>
