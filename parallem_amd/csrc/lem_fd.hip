@@ -207,9 +207,11 @@ __device__ __forceinline__ void slot_hist_flush(int *hist, DemScal *ds, int off,
 // Slopes are written only for downhill neighbours (sticky: Slopes keep the first pass's values); pass 2
 // passes slopes == nullptr.  Cells without a downhill neighbour are appended
 // to the plateau candidate list (the cells whose initial distance is 2e7).
-// A 64 x 16 tile per 64 x 4 block: the tile's surface plus a one-cell ring is
-// staged in LDS with every load in flight at once, then each thread evaluates
-// four rows from LDS.  Off-grid ring cells hold NaN: every comparison with it
+// A 64 x 4 block covers 64 x 64 cells in four 64 x 16 tiles: each tile's
+// surface plus a one-cell ring is staged in LDS with every load in flight at
+// once, then each thread evaluates four rows from LDS.  The block appends its
+// candidates with one atomic at the end (appends from many blocks to one
+// counter serialise).  Off-grid ring cells hold NaN: every comparison with it
 // is false, so such a neighbour is neither downhill, up nor flat, exactly as
 // a skipped one.
 template <typename TS>
@@ -224,70 +226,77 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
     __shared__ int s_wave[4];
     __shared__ int s_base;
     const int tid = threadIdx.y * 64 + threadIdx.x;
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 16;
-    double v[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int k = tid + q * 256;
-        int gy = y0 - 1 + k / RW, gx = x0 - 1 + k % RW;
-        v[q] = lm_nan();
-        if (k < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) v[q] = z[(long)gy * W + gx];
-    }
-    uint8_t mk[4];
-#pragma unroll
-    for (int rg = 0; rg < 4; rg++) {
-        int irow = y0 + rg * 4 + threadIdx.y, icol = x0 + threadIdx.x;
-        mk[rg] = (icol < W && irow < H) ? mask[(long)irow * W + icol] : 0;
-    }
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int k = tid + q * 256;
-        if (k < RN) zs[k] = v[q];
-    }
-    __syncthreads();
+    const int x0 = blockIdx.x * 64;
     const int icol = x0 + threadIdx.x;
-    unsigned cand_rows = 0;   // bit rg: the cell of row group rg is a plateau candidate
-#pragma unroll 1
-    for (int rg = 0; rg < 4; rg++) {
-        int lr = rg * 4 + threadIdx.y;
-        int irow = y0 + lr;
-        bool inside = icol < W && irow < H;
-        long self = (long)irow * W + icol;
-        bool active = inside && mk[rg];
-        int aspect = 0, aspectsfd = 0;
-        unsigned eq = 0;
-        if (active) {
-            const int c0 = (lr + 1) * RW + (threadIdx.x + 1);
-            double zc = zs[c0];
-            double smax = 0.0;
-            int up = 0, flat = 0;
+    unsigned cand_rows = 0;   // bit 4 * sub + rg: the cell of row group rg of tile sub is a plateau candidate
+    for (int sub = 0; sub < 4; sub++) {
+        const int y0 = blockIdx.y * 64 + sub * 16;
+        if (y0 >= H) break;
+        double v[PER];
 #pragma unroll
-            for (int d = 0; d < 8; d++) {
-                float dc = (d & 1) ? 1.41f : 1.0f;
-                float den = (float)cell_size * dc;
-                double zn = zs[c0 + kDY[d] * RW + kDX[d]];
-                double stemp = (zc - zn) / (double)den;
-                if ((zc > zn) && (zn != -9999)) {
-                    if (slopes) slopes[d * plane + self] = (TS)stemp;
-                    if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
-                    aspect += kCODE[d];
-                }
-                if (zc < zn) up++;
-                if (zc == zn) { flat++; eq |= 1u << d; }
-            }
-            if (up > 7) { aspect = 0; aspectsfd = 0; }
-            if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
-            if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
+        for (int q = 0; q < PER; q++) {
+            int k = tid + q * 256;
+            int gy = y0 - 1 + k / RW, gx = x0 - 1 + k % RW;
+            v[q] = lm_nan();
+            if (k < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) v[q] = z[(long)gy * W + gx];
         }
-        if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; eqb[self] = (uint8_t)eq; }
-        if (active && aspect == 0 && aspectsfd == 0) cand_rows |= 1u << rg;
+        uint8_t mk[4];
+#pragma unroll
+        for (int rg = 0; rg < 4; rg++) {
+            int irow = y0 + rg * 4 + threadIdx.y, icol = x0 + threadIdx.x;
+            mk[rg] = (icol < W && irow < H) ? mask[(long)irow * W + icol] : 0;
+        }
+        __syncthreads();   // the previous tile's reads are done
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int k = tid + q * 256;
+            if (k < RN) zs[k] = v[q];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int rg = 0; rg < 4; rg++) {
+            int lr = rg * 4 + threadIdx.y;
+            int irow = y0 + lr;
+            bool inside = icol < W && irow < H;
+            long self = (long)irow * W + icol;
+            bool active = inside && mk[rg];
+            int aspect = 0, aspectsfd = 0;
+            unsigned eq = 0;
+            if (active) {
+                const int c0 = (lr + 1) * RW + (threadIdx.x + 1);
+                double zc = zs[c0];
+                double smax = 0.0;
+                int up = 0, flat = 0;
+#pragma unroll
+                for (int d = 0; d < 8; d++) {
+                    float dc = (d & 1) ? 1.41f : 1.0f;
+                    float den = (float)cell_size * dc;
+                    double zn = zs[c0 + kDY[d] * RW + kDX[d]];
+                    double stemp = (zc - zn) / (double)den;
+                    if ((zc > zn) && (zn != -9999)) {
+                        if (slopes) slopes[d * plane + self] = (TS)stemp;
+                        if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
+                        aspect += kCODE[d];
+                    }
+                    if (zc < zn) up++;
+                    if (zc == zn) { flat++; eq |= 1u << d; }
+                }
+                if (up > 7) { aspect = 0; aspectsfd = 0; }
+                if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
+                if ((flat < 8) && (flat + up > 7)) { aspect = 0; aspectsfd = 0; }
+            }
+            if (inside) { fd[self] = (uint8_t)aspect; SFD[self] = (uint8_t)aspectsfd; eqb[self] = (uint8_t)eq; }
+            if (active && aspect == 0 && aspectsfd == 0) cand_rows |= 1u << (sub * 4 + rg);
+        }
     }
-    // one append per block for its candidates (row-major order within the block is not needed)
+    // one append per block for its candidates (their order is not needed)
     int mine = __popc(cand_rows);
     int base = block_append_n(ncand, mine, s_wave, &s_base);
-#pragma unroll
-    for (int rg = 0; rg < 4; rg++)
-        if ((cand_rows >> rg) & 1) cand[base++] = (int)((long)(y0 + rg * 4 + threadIdx.y) * W + icol);
+    while (cand_rows) {
+        const int b = __ffs(cand_rows) - 1;
+        cand_rows &= cand_rows - 1;
+        cand[base++] = (int)((long)(blockIdx.y * 64 + (b >> 2) * 16 + (b & 3) * 4 + threadIdx.y) * W + icol);
+    }
 }
 
 // ------------------------------------------------------------ plateaus
@@ -1027,7 +1036,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     uint8_t *eqb = c->dmask;   // equal-height neighbour bits; the accumulation passes reuse the buffer later
     LEM_CHECK(c, hipMemsetAsync(&c->scal->ncand, 0, sizeof(int), c->s));
     dim3 blk(64, 4);
-    dim3 grd((W + 63) / 64, (H + 15) / 16);
+    dim3 grd((W + 63) / 64, (H + 63) / 64);
     prof_begin(c, PROF_FLOWDIRS);
     k_flowdirs<TS><<<grd, blk, 0, c->s>>>(W, H, (int)c->p.cellsize, zs, c->mask, slopes, c->NP, c->fd, c->SFD, eqb,
                                           c->cand, &c->scal->ncand);
