@@ -568,7 +568,8 @@ __global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *S
     if (c >= W) return;
     for (int r = blockIdx.y; r < H; r += gridDim.y) {
         const long i = (long)r * W + c;
-        flag[i] = (mask[i] && SFD[i] == 0 && !dem_border(W, H, r, c, i, demid, dd)) ? 1 : 0;
+        const uint8_t m = mask[i], s = SFD[i];   // both loads in flight
+        flag[i] = (m && s == 0 && !dem_border(W, H, r, c, i, demid, dd)) ? 1 : 0;
     }
 }
 
@@ -580,14 +581,16 @@ __global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD,
     if (c >= W) return;
     for (int r = blockIdx.y; r < H; r += gridDim.y) {
         const long i = (long)r * W + c;
+        const uint8_t m = mask[i], s = SFD[i];   // loads in flight together
+        const int sl = slot_of(demid, i);
         int v;
-        if (!mask[i]) v = 0;
+        if (!m) v = 0;
         else if (dem_border(W, H, r, c, i, demid, dd)) v = 0;
-        else if (SFD[i] == 0) {
+        else if (s == 0) {
             v = prefix[i] + 1;
-            labdem[v] = (uint8_t)slot_of(demid, i);
+            labdem[v] = (uint8_t)sl;
         } else {
-            int d = code_dir(SFD[i]);
+            int d = code_dir(s);
             v = -((r + kDY[d]) * W + (c + kDX[d]) + 1);
         }
         ws[i] = v;
@@ -965,9 +968,11 @@ __global__ void k_raise(long N, const uint8_t *mask, const int *ws, const unsign
     const unsigned long long INF = ord_enc(lm_inf());
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         double v = z[i];
+        const uint8_t m = mask[i];   // the cell's own loads in flight together
+        const int lab = ws[i];
+        const int sl = slot_of(demid, i);
         // floodingDriver only runs for a DEM with sinks left (routeflow.cu:913)
-        if (mask[i] && ds[slot_of(demid, i)].sinks_pass1 > 0) {
-            int lab = ws[i];
+        if (m && ds[sl].sinks_pass1 > 0) {
             unsigned long long e = r[lab];
             double rs = (lab == 0 || e == INF) ? 0.0 : ord_dec(e);
             if (v < rs) v = rs;
