@@ -860,12 +860,16 @@ __global__ void k_minimax_relax(int ne, const uint32_t *elo, const uint32_t *ehi
 // has not reached yet can lower nothing, and the bits (one word per 32
 // labels) stay in the level-2 cache where the values do not.  A bit read
 // before another edge sets it in the same pass only defers that edge to the
-// next pass.
+// next pass.  Passes alternate the direction in which the grid walks the
+// label-sorted edge list: workgroups run roughly in order, and a value lowered
+// early in a pass is often seen later in the same pass, so chains of basins
+// whose labels fall are carried as far per pass as chains whose labels rise.
 __global__ void k_minimax_relax_reached(int ne, const uint32_t *elo, const uint32_t *ehi, const unsigned long long *ew,
-                                        unsigned long long *r, uint32_t *reached, int *flag)
+                                        unsigned long long *r, uint32_t *reached, int *flag, int rev)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ne) return;
+    if (rev) k = ne - 1 - k;
     uint32_t a = elo[k], b = ehi[k];
     bool ba = (reached[a >> 5] >> (a & 31)) & 1u, bb = (reached[b >> 5] >> (b & 31)) & 1u;
     if (!ba && !bb) return;
@@ -1343,7 +1347,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
             if (na > 0)
                 for (int q = 0; q < 16; q++)
                     k_minimax_relax_reached<<<grid1d(na, 256, 1 << 30), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, reached,
-                                                                                         &c->scal->flag);
+                                                                                         &c->scal->flag, q & 1);
             LEM_LAUNCHED(c);
             uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
             unsigned long long *w2 = c->ew2[buf];
