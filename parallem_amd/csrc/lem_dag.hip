@@ -606,32 +606,37 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
                         word = wd[cur];   // only dead-count and counter bits of a dead seed can have moved
                     }
                     const bool dead = Cell::kCanDie && ((word >> 24) & 0xFu) != 0;
+                    // receivers: the set bits of the fd code (bit k is direction
+                    // (k + 2) & 7); the first one's address is ready before the
+                    // wait below.  A cell with no receiver adds 0 to its own
+                    // word instead, whose counter is not 1 once it is resolved
+                    // (0, or what is left of W_NEVER), so nothing is handed over
+                    unsigned f = (word >> 16) & 0xFFu;
+                    int rp0 = f ? cur + koff<TW>(__ffs(f) - 1) : cur;
                     uint32_t ns = ST_DEAD;
                     if (!dead) {
                         cell.compute(lds, cur, (word >> 8) & 0xFFu, (word & W_UPD) != 0);
                         ns = ST_DONE;
                     }
                     wd[cur] = (word & ~(3u << W_STSH)) | (ns << W_STSH);
+                    asm volatile("" : "+v"(rp0));
                     lds_order();
                     // unsigned adds: -1, or -1 plus one dead donor (the counter
                     // of a receiver is at least 1 here, so no borrow leaves it)
                     const uint32_t delta = dead ? (W_DEAD1 - 1u) : 0xFFFFFFFFu;
-                    unsigned f = (word >> 16) & 0xFFu;
-                    int nxt = -1;
-                    uint32_t nword = 0;
-                    // receivers: the set bits of the fd code (bit k is direction (k + 2) & 7)
-                    if (f) {
+                    int nxt;
+                    uint32_t nword;
+                    {
                         // the first (under single flow directions the only)
                         // receiver, without branches: the walk continues with
                         // it when the add hands it over
-                        const int k = __ffs(f) - 1;
+                        const uint32_t d0 = f ? delta : 0u;
                         f &= f - 1;
-                        const int rp = cur + koff<TW>(k);
-                        const uint32_t old = atomicAdd(&wd[rp], delta);
+                        const uint32_t old = atomicAdd(&wd[rp0], d0);
                         // a ring word carries the neighbour tile's position
                         my_exits |= (((old >> W_STSH) & 3u) == 3u) ? ((old >> 8) & 0x1FFu) : 0u;
-                        nxt = ((old & W_CNT) == 1u) ? rp : -1;
-                        nword = old + delta;
+                        nxt = ((old & W_CNT) == 1u) ? rp0 : -1;
+                        nword = old + d0;
                     }
                     while (!SFD && f) {   // further receivers (multiple flow directions)
                         const int k = __ffs(f) - 1;
