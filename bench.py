@@ -57,9 +57,9 @@ def kernel_bytes(kernel, t):
                geli, stone t each + soilT 8 read; depo, stone, fines t each +
                soilT 8 written.
     mod2       (k_mod2): fa 8, ero, inci, geli, depo, temp, rain, last_temp,
-               stone, fines, soilM, soilB, nut, TotB t each, soilT 8, dem 8
-               read; dz, wC, wP, stone, fines, soilM, soilB, nut, TotB t each,
-               soilT 8, dem 8 written.
+               stone, fines, soilM, soilB, nut, TotB and the two calcite
+               terms t each, soilT 8, dem 8 read; dz, wC, wP, stone, fines,
+               soilM, soilB, nut, TotB t each, soilT 8, dem 8 written.
     mod1       (k_mod1): Slopes 8t, fa 8, fines, nut, stone, TotB, temp t each,
                soilT 8 read; SlopePtr, ero, inci, geli, depo t each written.
     flowdirs   (k_flowdirs): dem 8 (stencil, read once from HBM), mask 1 read;
@@ -68,7 +68,7 @@ def kernel_bytes(kernel, t):
     return {
         "fa_rounds": 2 + t + 8,
         "sed_rounds": 2 + 4 * t + 8 + 3 * t + 8,
-        "mod2": 8 + 13 * t + 8 + 8 + 9 * t + 8 + 8,
+        "mod2": 8 + 15 * t + 8 + 8 + 9 * t + 8 + 8,
         "mod1": 8 * t + 8 + 5 * t + 8 + 5 * t,
         "flowdirs": 8 + 1 + 2 + 8 * t,
     }.get(kernel)

@@ -185,7 +185,7 @@ static void free_all(lemhip_ctx *c)
                     c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
                     c->dmask, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
                     c->dag_ctl, c->partials, c->scal, c->demid, c->ddesc, c->dsc, c->slot_flags, c->labdem,
-                    c->dstage};
+                    c->dstage, c->tcache};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int q = 0; q < 2; q++) {
@@ -269,6 +269,7 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     LEM_CHECK(c, hipMemset(c->pl_tiles, 0, 6 * ptiles * 4));
     LEM_TRY(A((void **)&c->cand, N * 4));
     LEM_TRY(A((void **)&c->ws, N * 4));
+    LEM_TRY(A(&c->tcache, 2 * N * T));
     LEM_TRY(A((void **)&c->scan_tmp, N * 4));
     LEM_TRY(A((void **)&c->z2, N * 8));
     LEM_TRY(A((void **)&c->dmask, N));
@@ -424,6 +425,7 @@ static void launch_init_rect(lemhip_ctx *c, const DemDesc &D, int slot, const do
 // matrices) for one DEM rectangle
 static int init_slot(lemhip_ctx *c, int slot, const double *dem, const double *rain, int soilT_mode)
 {
+    c->tcache_ok = false;   // the slot's temperatures are set
     DemDesc D;
     LEM_TRY(slot_rect(c, slot, 1, &D));
     long n = (long)D.w * D.h;
@@ -617,6 +619,7 @@ static int slot_download(lemhip_ctx *c, int slot, int field, void *host)
 
 static int slot_upload(lemhip_ctx *c, int slot, int field, const void *host)
 {
+    c->tcache_ok = false;   // any field: temperatures among them
     int kind = -1;
     void *dst = field_ptr(c, field, &kind);
     if (!dst) { c->err = "unknown field"; return LEMHIP_E_ARG; }
@@ -925,6 +928,7 @@ extern "C" int lemhip_run(lemhip_ctx *c, int iter0, int nsteps)
 
 static int set_climate_rect(lemhip_ctx *c, const DemDesc &D, double rain, double temp)
 {
+    c->tcache_ok = false;
     long n = (long)D.w * D.h;
     if (c->prec == LEMHIP_FP64)
         k_set_climate<double><<<grid1d(n), 256, 0, c->s>>>(c->W, D, rain, temp, c->dem, c->mask,

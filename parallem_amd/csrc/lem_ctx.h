@@ -119,6 +119,8 @@ struct lemhip_ctx {
     int dag_rounds_last = 0;
     int plat_k = 8;             // plateau rounds per launch (4, 8 or 16)
     bool tune_pending = false;  // the next step times the candidates and keeps the fastest
+    void *tcache = nullptr;     // MOD part 2's calcite terms per cell (2 x N, T), from the temperatures
+    bool tcache_ok = false;     // false after anything that may change a temperature
     double tune_ms[16] = {};    // last tuning: per-candidate times
     int dag_shape = 0;          // tile shape id of the dependency engine
     int dag_batch = 16;         // round launches between host checks

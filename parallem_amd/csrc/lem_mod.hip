@@ -161,6 +161,40 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
 // Update_Surface (updates.cu:23-46), update_nutrient_content (:48-76) and
 // grow_vegetation (:78-139): each reads only its own cell, so one pass keeps
 // the reference's per-cell order.
+// The calcite equilibrium term of weathering type 1 (eroincidep.cu:220-327)
+// depends only on a cell's temperature and on which side of 0.1 its soil
+// thickness lies: both variants are computed here, with the same expressions
+// as the body written in one piece, whenever the temperatures change
+// (initialisation, climate updates, uploads), and MOD part 2 reads them.
+template <typename T>
+__global__ void __launch_bounds__(256) k_calcite_terms(long N, const uint8_t *__restrict__ mask,
+                                                       const T *__restrict__ temp, T *__restrict__ ca_hi,
+                                                       T *__restrict__ ca_lo)
+{
+    for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
+        if (mask[self] == 0) continue;
+        T tc = temp[self];
+        T t_abs = tc + (T)273.16;
+        T t2 = t_abs * t_abs;
+        T ltemp = mlog10(t_abs);
+        T DHA = (T)-0.4883 + (T)8.074e-4 * tc;
+        T DHB = (T)-0.3241 + (T)1.600e-4 * tc;
+        T sqI = (T)0.31622776601683794; // sqrt(0.1), correctly rounded
+        T denom1 = ((T)-4. * DHA * sqI) / ((T)1. + (T)5.e-8 * DHB * sqI);
+        T gammCa_d = mpow((T)10., denom1);
+        T gammHCO3_d = mpow((T)10., (-DHA * sqI) / ((T)1. + (T)5.4e-8 * DHB * sqI));
+        T K1_d = mpow((T)10., (T)-356.3094 - (T)0.06091964 * t_abs + (T)21834.37 / t_abs + (T)126.8339 * ltemp - (T)1684915 / t2);
+        T K2_d = mpow((T)10., (T)-107.8871 - (T)0.03252849 * t_abs + (T)5151.79 / t_abs + (T)38.92561 * ltemp - (T)563713.9 / t2);
+        T KC_d = mpow((T)10., (T)-171.9065 - (T)0.077993 * t_abs + (T)2839.319 / t_abs + (T)71.595 * ltemp);
+        T KH_d = mpow((T)10., (T)108.3865 + (T)0.01985076 * t_abs - (T)6919.53 / t_abs - (T)40.45154 * ltemp + (T)669365.0 / t2);
+        // PCO2 for soil thicker than 0.1, and for the rest
+        T PCO2hi = mpow((T)10., (T)-2.5 + ((tc - (T)15.) / (T)10.));
+        T PCO2lo = mpow((T)10., (T)-3.5 + ((tc - (T)15.) / (T)10.));
+        ca_hi[self] = mpow(PCO2hi * ((K1_d * KC_d * KH_d) / ((T)4. * K2_d * gammCa_d * gammHCO3_d * gammHCO3_d)), ((T)1. / (T)3.));
+        ca_lo[self] = mpow(PCO2lo * ((K1_d * KC_d * KH_d) / ((T)4. * K2_d * gammCa_d * gammHCO3_d * gammHCO3_d)), ((T)1. / (T)3.));
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_t *__restrict__ mask,
                                               const typename FaS<T>::type *__restrict__ fa,
@@ -172,7 +206,8 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
                                               T *__restrict__ soilB, T *__restrict__ nut, T *__restrict__ TotB,
                                               double *__restrict__ dem, T *__restrict__ dzo, T *__restrict__ wCo,
                                               T *__restrict__ wPo, const uint8_t *__restrict__ demid,
-                                              const DemScal *__restrict__ ds)
+                                              const DemScal *__restrict__ ds, const T *__restrict__ ca_hi,
+                                              const T *__restrict__ ca_lo)
 {
     const double rescale_w = 10000 * (10000 / (p.cellsize * p.cellsize));
     const double rescale_v = (10000 / (p.cellsize * p.cellsize));
@@ -189,28 +224,12 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
         T rn = rain[self];
         T PE_d = (T)1.44 * mpow(tc + (T)6., (T)2);
         T P0_d = (T)7.7e-5, P1_d = (T)2.3;
-        T t_abs = tc + (T)273.16;
-        T t2 = t_abs * t_abs;
-        T ltemp = mlog10(t_abs);
-        T DHA = (T)-0.4883 + (T)8.074e-4 * tc;
-        T DHB = (T)-0.3241 + (T)1.600e-4 * tc;
-        T sqI = (T)0.31622776601683794; // sqrt(0.1), correctly rounded
-        T denom1 = ((T)-4. * DHA * sqI) / ((T)1. + (T)5.e-8 * DHB * sqI);
-        T gammCa_d = mpow((T)10., denom1);
-        T gammHCO3_d = mpow((T)10., (-DHA * sqI) / ((T)1. + (T)5.4e-8 * DHB * sqI));
-        T K1_d = mpow((T)10., (T)-356.3094 - (T)0.06091964 * t_abs + (T)21834.37 / t_abs + (T)126.8339 * ltemp - (T)1684915 / t2);
-        T K2_d = mpow((T)10., (T)-107.8871 - (T)0.03252849 * t_abs + (T)5151.79 / t_abs + (T)38.92561 * ltemp - (T)563713.9 / t2);
-        T KC_d = mpow((T)10., (T)-171.9065 - (T)0.077993 * t_abs + (T)2839.319 / t_abs + (T)71.595 * ltemp);
-        T KH_d = mpow((T)10., (T)108.3865 + (T)0.01985076 * t_abs - (T)6919.53 / t_abs - (T)40.45154 * ltemp + (T)669365.0 / t2);
         T surface_flow = (T)fa_val(fa[self]) * (T)1000;
         T evap_trans;
         if (rn + surface_flow <= PE_d) evap_trans = rn + surface_flow;
         else evap_trans = rn + surface_flow - PE_d;
         double sT = soilT[self];
-        T PCO2;
-        if (sT > 0.1) PCO2 = mpow((T)10., (T)-2.5 + ((tc - (T)15.) / (T)10.));
-        else PCO2 = mpow((T)10., (T)-3.5 + ((tc - (T)15.) / (T)10.));
-        T Ca2eq = mpow(PCO2 * ((K1_d * KC_d * KH_d) / ((T)4. * K2_d * gammCa_d * gammHCO3_d * gammHCO3_d)), ((T)1. / (T)3.));
+        T Ca2eq = (sT > 0.1) ? ca_hi[self] : ca_lo[self];
         T wC = (T)0.0148148148 * Ca2eq * (surface_flow / (T)rescale_w);
         if (wC <= (T)0.0) wC = 0;
         if (wC > (T)0.001) wC = (T)0.001;
@@ -603,12 +622,18 @@ static int mod_t(lemhip_ctx *c)
     prof_end(c, PROF_SED, c->dag_rounds_last);
     LEM_TRY(slot_sum<T>(c, c->depo, DS_OFF(totD)));
     k_slots_base<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
+    T *ca_hi = reinterpret_cast<T *>(c->tcache), *ca_lo = ca_hi + N;
+    if (!c->tcache_ok) {
+        k_calcite_terms<T><<<g, 256, 0, c->s>>>(N, c->mask, (const T *)c->temp, ca_hi, ca_lo);
+        LEM_LAUNCHED(c);
+        c->tcache_ok = true;
+    }
     prof_begin(c, PROF_MOD2);
     k_mod2<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const FS *)c->fa, (const T *)c->ero, (const T *)c->inci,
                                    (const T *)c->geli, (const T *)c->depo, (const T *)c->temp, (const T *)c->rain,
                                    (const T *)c->last_temp, c->soilT, (T *)c->stone, (T *)c->fines, (T *)c->soilM,
                                    (T *)c->soilB, (T *)c->nut, (T *)c->TotB, c->dem, (T *)c->dz, (T *)c->wC,
-                                   (T *)c->wP, c->demid, c->dsc);
+                                   (T *)c->wP, c->demid, c->dsc, ca_hi, ca_lo);
     prof_end(c, PROF_MOD2);
     k_slots_post<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
     LEM_LAUNCHED(c);

@@ -28,7 +28,7 @@ import numpy as np
 from . import lemhip, synth
 
 # device bytes per atlas cell of the two precision forms (state + scratch)
-BYTES_PER_CELL = {lemhip.FP64: 293, lemhip.FP32_KAHAN: 185}
+BYTES_PER_CELL = {lemhip.FP64: 309, lemhip.FP32_KAHAN: 193}
 
 
 @dataclasses.dataclass

@@ -152,3 +152,34 @@ def test_gpu_math_bit_identical_to_host():
         g = lemhip.math_eval(fn, arg, arg2)
         h = O.math_eval(fn, arg, arg2)
         assert _same(g, h), f"math fn {fn} differs"
+
+
+def test_temperature_change_mid_run():
+    """MOD part 2 keeps per-cell terms computed from the temperatures; a new
+    temperature field (upload) or a new climate (set_climate) between steps
+    must be picked up, bit for bit with the oracle given the same change."""
+    from oracle import oracle as O
+    dem = synth.config_dem(1, side=128, seed_offset=11)
+    ctx, orc = _pair(dem)
+    it = -1000
+    for k in range(2):
+        ctx.step(it + k); orc.step(it + k)
+    # a warmer, uneven field through an upload
+    t = orc["temp"].copy()
+    valid = orc["mask"] == 1
+    t[valid] += 3.0 + 0.01 * (np.arange(t.size)[valid] % 97)
+    orc["temp"][:] = t
+    ctx.upload("temp", t.reshape(dem.z.shape))
+    for k in range(2, 4):
+        ctx.step(it + k); orc.step(it + k)
+        _assert_fields(ctx, orc, f"after upload, step {k}", fields=["dem", "soilT", "stone", "wC", "wP"])
+    # a new climate through the setter (the oracle's own climate routine)
+    ctx.set_climate(1.5, 12.0)
+    p = O.default_params(*dem.z.shape[::-1])
+    orc.L.orc_SetClimateMatrices(O.C.byref(p), O.iptr(orc["mask"]), O.dptr(orc["dem"]), 1.5, 12.0,
+                                 O.dptr(orc["rain"]), O.dptr(orc["temp"]))
+    _assert_fields(ctx, orc, "after set_climate", ints=("mask",), fields=["rain", "temp"])
+    for k in range(4, 6):
+        ctx.step(it + k); orc.step(it + k)
+        _assert_fields(ctx, orc, f"after set_climate, step {k}", fields=["dem", "soilT", "stone", "wC", "wP"])
+    _assert_summary(ctx, orc, "final")
