@@ -285,12 +285,21 @@ __global__ void __launch_bounds__(256) k_fa_prep(int W, int H, const uint8_t *__
     __shared__ uint8_t smk[PREP_RH * PREP_RW], sfd[PREP_RH * PREP_RW];
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * (4 * PREP_RG);
-    for (int q = tid; q < PREP_RH * PREP_RW; q += 256) {
-        int gy = y0 - 1 + q / PREP_RW, gx = x0 - 1 + q % PREP_RW;
-        bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-        long n = (long)gy * W + gx;
-        smk[q] = in ? mask[n] : 0;
-        sfd[q] = in ? fd[n] : 0;
+    constexpr int SN = PREP_RH * PREP_RW, SPER = (SN + 255) / 256;
+    uint8_t vm[SPER], vf[SPER];
+#pragma unroll
+    for (int j = 0; j < SPER; j++) {
+        const int q = tid + j * 256;
+        const int gy = y0 - 1 + q / PREP_RW, gx = x0 - 1 + q % PREP_RW;
+        const bool in = q < SN && gx >= 0 && gx < W && gy >= 0 && gy < H;
+        const long n = (long)gy * W + gx;
+        vm[j] = in ? mask[n] : 0;
+        vf[j] = in ? fd[n] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SPER; j++) {
+        const int q = tid + j * 256;
+        if (q < SN) { smk[q] = vm[j]; sfd[q] = vf[j]; }
     }
     __syncthreads();
 #pragma unroll
@@ -333,16 +342,31 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
     __shared__ T se[PREP_RH * PREP_RW], si[PREP_RH * PREP_RW], sg[PREP_RH * PREP_RW];
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * (4 * PREP_RG);
-    for (int q = tid; q < PREP_RH * PREP_RW; q += 256) {
-        int gy = y0 - 1 + q / PREP_RW, gx = x0 - 1 + q % PREP_RW;
-        bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-        long n = (long)gy * W + gx;
-        uint8_t m = in ? mask[n] : 0;
-        smk[q] = m;
-        sfd[q] = in ? fd[n] : 0;
-        se[q] = m ? ero[n] : (T)0;
-        si[q] = m ? inci[n] : (T)0;
-        sg[q] = m ? geli[n] : (T)0;
+    // every load of the region in flight before the first LDS store
+    constexpr int SN = PREP_RH * PREP_RW, SPER = (SN + 255) / 256;
+    uint8_t vm[SPER], vf[SPER];
+    T ve[SPER], vi[SPER], vg[SPER];
+#pragma unroll
+    for (int j = 0; j < SPER; j++) {
+        const int q = tid + j * 256;
+        const int gy = y0 - 1 + q / PREP_RW, gx = x0 - 1 + q % PREP_RW;
+        const bool in = q < SN && gx >= 0 && gx < W && gy >= 0 && gy < H;
+        const long n = (long)gy * W + gx;
+        vm[j] = in ? mask[n] : 0;
+        vf[j] = in ? fd[n] : 0;
+        ve[j] = in ? ero[n] : (T)0;
+        vi[j] = in ? inci[n] : (T)0;
+        vg[j] = in ? geli[n] : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < SPER; j++) {
+        const int q = tid + j * 256;
+        if (q >= SN) continue;
+        smk[q] = vm[j];
+        sfd[q] = vf[j];
+        se[q] = vm[j] ? ve[j] : (T)0;
+        si[q] = vm[j] ? vi[j] : (T)0;
+        sg[q] = vm[j] ? vg[j] : (T)0;
     }
     __syncthreads();
     const T one = (T)1;
