@@ -549,12 +549,19 @@ __global__ void __launch_bounds__(256) k_count_sinks(const int *cand, int n, con
 
 // DEM-local border of init_watershedMFD (newflow.cu:371): last column, first
 // and last row of the cell's own DEM
-__device__ __forceinline__ bool dem_border(int W, int H, int r, int c, long i, const uint8_t *demid, const DemDesc *dd)
+// A cell of a DEM lies in its last column, first row or last row exactly when
+// its east, north or south neighbour is outside the DEM's rectangle: off the
+// grid, or in another DEM's rectangle or none (rectangles are convex).  The
+// three slot ids are read at once instead of the DEM's rectangle after the
+// cell's own id.
+__device__ __forceinline__ bool dem_border(int W, int H, int r, int c, long i, const uint8_t *demid)
 {
     if (demid) {
-        DemDesc D = dd[demid[i]];
-        r -= D.y0; c -= D.x0;
-        return (c == D.w - 1) || (r == 0) || (r == D.h - 1);
+        const uint8_t me = demid[i];
+        const uint8_t e = c + 1 < W ? demid[i + 1] : DEM_NONE;
+        const uint8_t n = r > 0 ? demid[i - W] : DEM_NONE;
+        const uint8_t s = r + 1 < H ? demid[i + W] : DEM_NONE;
+        return e != me || n != me || s != me;
     }
     return (c == W - 1) || (r == 0) || (r == H - 1);
 }
@@ -569,7 +576,7 @@ __global__ void k_sink_flags(int W, int H, const uint8_t *mask, const uint8_t *S
     for (int r = blockIdx.y; r < H; r += gridDim.y) {
         const long i = (long)r * W + c;
         const uint8_t m = mask[i], s = SFD[i];   // both loads in flight
-        flag[i] = (m && s == 0 && !dem_border(W, H, r, c, i, demid, dd)) ? 1 : 0;
+        flag[i] = (m && s == 0 && !dem_border(W, H, r, c, i, demid)) ? 1 : 0;
     }
 }
 
@@ -585,7 +592,7 @@ __global__ void k_init_ws(int W, int H, const uint8_t *mask, const uint8_t *SFD,
         const int sl = slot_of(demid, i);
         int v;
         if (!m) v = 0;
-        else if (dem_border(W, H, r, c, i, demid, dd)) v = 0;
+        else if (dem_border(W, H, r, c, i, demid)) v = 0;
         else if (s == 0) {
             v = prefix[i] + 1;
             labdem[v] = (uint8_t)sl;
