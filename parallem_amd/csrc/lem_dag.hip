@@ -381,9 +381,17 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
         unsigned bits = 0;
         int s = 0;
         if (smk[p]) {
-            int dr = irow, dc = icol;
-            if (demid) { DemDesc D = dd[demid[i]]; dr -= D.y0; dc -= D.x0; }
-            bool dead = (dr == 0 || dc == 0);
+            // row 0 or column 0 of the cell's DEM: the north or west
+            // neighbour is outside the DEM's rectangle (off the grid, or in
+            // another DEM's rectangle or none)
+            bool dead;
+            if (demid) {
+                const uint8_t me = demid[i];
+                const uint8_t nn = irow > 0 ? demid[i - W] : DEM_NONE, ww = icol > 0 ? demid[i - 1] : DEM_NONE;
+                dead = nn != me || ww != me;
+            } else {
+                dead = irow == 0 || icol == 0;
+            }
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int d = cpull(k);
