@@ -739,8 +739,11 @@ __global__ void k_count_pending(long N, const uint8_t *mask, const int *st, int 
 {
     __shared__ int sh[256];
     int acc = 0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
-        acc += (mask[i] && (st[i] == DAG_PENDING || st[i] < 0)) ? 1 : 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        const uint8_t m = mask[i];   // both loads in flight together
+        const int s = st[i];
+        acc += (m && (s == DAG_PENDING || s < 0)) ? 1 : 0;
+    }
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {

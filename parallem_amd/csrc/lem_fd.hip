@@ -657,16 +657,26 @@ __global__ void __launch_bounds__(256) k_jump_list(long N, const uint8_t *mask, 
                                   [&](long i, int at) { list[at] = (int)i; });
 }
 
+// One lane per wave raises a progress flag (every lane writing the same word
+// serialises those writes).
+__device__ __forceinline__ void wave_flag(bool f, int *flag)
+{
+    const unsigned long long m = __ballot(f);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) *flag = 1;
+}
+
 __global__ void k_jump_some(int n, const int *list, int *ws, int *flag)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int i = list[k];
         int v = ws[i];
+        bool more = false;
         if (v < 0) {
             int t = ws[-v - 1];
             ws[i] = t;
-            if (t < 0) *flag = 1;
+            more = t < 0;
         }
+        wave_flag(more, flag);
     }
 }
 
@@ -676,11 +686,13 @@ __global__ void k_jump(long N, const uint8_t *mask, int *ws, int *flag)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         int v = ws[i];
+        bool more = false;
         if (v < 0 && mask[i]) {
             int t = ws[-v - 1];
             ws[i] = t;
-            if (t < 0) *flag = 1;
+            more = t < 0;
         }
+        wave_flag(more, flag);
     }
 }
 
@@ -887,13 +899,12 @@ __global__ void k_minimax_relax_reached(int ne, const uint32_t *elo, const uint3
     if (ca < ra) {
         atomicMin(&r[a], ca);
         if (!ba) atomicOr(&reached[a >> 5], 1u << (a & 31));
-        *flag = 1;
     }
     if (cb < rb) {
         atomicMin(&r[b], cb);
         if (!bb) atomicOr(&reached[b >> 5], 1u << (b & 31));
-        *flag = 1;
     }
+    wave_flag(ca < ra || cb < rb, flag);
 }
 
 // An edge with weight >= the values of both its labels can never lower
