@@ -712,15 +712,15 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
 // edge list keeps the minimum per unordered pair (order-free, exact).  The
 // cells of a block share most of their basin pairs (a boundary runs through
 // many cells): pairs are first merged in an LDS table (minimum weight per
-// pair), and each block appends its distinct pairs to a list with one atomic
-// (blocks of 64 x 32 cells: appends from many blocks to one counter
-// serialise, so a block covers many cells); pairs that find no LDS slot are
-// appended one by one.  The list is then
-// sorted by pair and reduced to the minimum weight per pair.  Keys pack
+// pair), and each block appends its distinct pairs to a list with one atomic;
+// pairs that find no LDS slot are appended one by one.  Blocks of 64 x 16
+// cells with a 1024-entry table (16 KB) keep eight blocks resident per
+// compute unit.  The list is then sorted by pair and reduced to the minimum
+// weight per pair.  Keys pack
 // (lo << B) | hi with B bits per label.  The count may exceed the capacity:
 // the caller then grows the list and runs again.
-#define EDGE_LDS 2048
-#define EDGE_RG 8   // row groups of 4 per block: 64 x (4 * EDGE_RG) cells
+#define EDGE_LDS 1024
+#define EDGE_RG 4   // row groups of 4 per block: 64 x (4 * EDGE_RG) cells
 __global__ void __launch_bounds__(256) k_edges_emit(int W, int H, const uint8_t *__restrict__ mask,
                                                     const int *__restrict__ ws, const double *__restrict__ z,
                                                     const uint8_t *__restrict__ demid, unsigned long long *keys,
