@@ -21,9 +21,11 @@ struct ProfSlot {
     long events = 0;
 };
 
-// plateau tiles: PTW x PTH cells advanced plat_k synchronous rounds per launch
-#define PTW 64
-#define PTH 32
+// plateau tiles: PTW x PTH cells advanced plat_k synchronous rounds per launch.
+// At 8 rounds per launch a 48 x 24 tile has a 64 x 40 region: each wave holds
+// one region row and the 512 threads hold exactly 5 cells each.
+#define PTW 48
+#define PTH 24
 
 // per-cell status of the dependency engine
 enum : uint8_t { ST_PENDING = 0, ST_DONE = 1, ST_DEAD = 2, ST_RING_WAIT = 3 };
