@@ -204,7 +204,7 @@ __device__ __forceinline__ void slot_hist_flush(int *hist, DemScal *ds, int off,
 
 // newflow.cu:49-174.  One thread per cell; a wave covers 64 consecutive cells
 // of a row so the centre row and both neighbour rows are read coalesced.
-// Slopes are written only for downhill neighbours (sticky: Slopes keep the first pass's values); pass 2
+// Slopes take new values only for downhill neighbours (sticky: Slopes keep the first pass's values); pass 2
 // passes slopes == nullptr.  Cells without a downhill neighbour are appended
 // to the plateau candidate list (the cells whose initial distance is 2e7).
 // A 64 x 4 block covers 64 x 64 cells in four 64 x 16 tiles: each tile's
@@ -263,23 +263,39 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
             int aspect = 0, aspectsfd = 0;
             unsigned eq = 0;
             if (active) {
+                // Every Slopes plane of the cell is rewritten, keeping the old value
+                // where the neighbour is not downhill: whole cache lines are written
+                // instead of lines with holes in them.  The old values are loaded
+                // first and merged after the scan.
+                TS sv[8];
+                if (slopes) {
+#pragma unroll
+                    for (int d = 0; d < 8; d++) sv[d] = slopes[d * plane + self];
+                }
                 const int c0 = (lr + 1) * RW + (threadIdx.x + 1);
                 double zc = zs[c0];
                 double smax = 0.0;
                 int up = 0, flat = 0;
+                unsigned down = 0;
+                TS nv[8];
 #pragma unroll
                 for (int d = 0; d < 8; d++) {
                     float dc = (d & 1) ? 1.41f : 1.0f;
                     float den = (float)cell_size * dc;
                     double zn = zs[c0 + kDY[d] * RW + kDX[d]];
                     double stemp = (zc - zn) / (double)den;
+                    nv[d] = (TS)stemp;
                     if ((zc > zn) && (zn != -9999)) {
-                        if (slopes) slopes[d * plane + self] = (TS)stemp;
+                        down |= 1u << d;
                         if (stemp > smax) { smax = stemp; aspectsfd = kCODE[d]; }
                         aspect += kCODE[d];
                     }
                     if (zc < zn) up++;
                     if (zc == zn) { flat++; eq |= 1u << d; }
+                }
+                if (slopes) {
+#pragma unroll
+                    for (int d = 0; d < 8; d++) slopes[d * plane + self] = ((down >> d) & 1u) ? nv[d] : sv[d];
                 }
                 if (up > 7) { aspect = 0; aspectsfd = 0; }
                 if ((flat + up) > 7 && up <= 7) { aspect = 0; aspectsfd = 0; }
