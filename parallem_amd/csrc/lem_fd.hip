@@ -698,9 +698,6 @@ __global__ void k_jump_some(int n, const int *list, int *ws, int *flag)
 
 // identify_watershedMFD (newflow.cu:469-487): pointer jumping; in-place reads
 // of already-jumped targets only shorten chains
-#ifndef JUMP_FULL
-#define JUMP_FULL 4
-#endif
 __global__ void k_jump(long N, const uint8_t *mask, int *ws, int *flag)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
@@ -1290,7 +1287,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     // pointer jumping: four passes over every cell, then only the cells still pointing
     for (int it = 0;; it += 4) {
         LEM_TRY(clear_flag(c));
-        for (int q = 0; q < JUMP_FULL; q++) k_jump<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, &c->scal->flag);
+        for (int q = 0; q < 4; q++) k_jump<<<grid1d(N), 256, 0, c->s>>>(N, c->mask, c->ws, &c->scal->flag);
         LEM_LAUNCHED(c);
         int f = 0;
         LEM_TRY(read_flag(c, &f));
