@@ -85,7 +85,7 @@ def rocprof_name(slot, tuning):
 
 def dems(count):
     from parallem_amd import synth
-    return [synth.config_dem(2, seed_offset=k, side=SIDE) for k in range(count)]
+    return [synth.fbm_4096(seed_offset=k, side=SIDE) for k in range(count)]
 
 
 def timed(step, steps, warmup, handle, autotune=True):
@@ -190,7 +190,7 @@ def measured_traffic(kernel, tuning, batch):
 def cpu_baseline(side, threads):
     from oracle import oracle as O
     from parallem_amd import synth
-    dem = synth.config_dem(2, side=side)
+    dem = synth.fbm_4096(side=side)
     O.lib(True).orc_set_threads(threads)
     o = O.Oracle(dem.z, dem.rain, O.default_params(side, side), openmp=True, soilT_mode="nonneg")
     t0 = time.perf_counter()

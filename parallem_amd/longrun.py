@@ -38,7 +38,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     side = a.side
     t0 = time.perf_counter()
-    dem = synth.config_dem(5, seed_offset=a.seed_offset, side=side)
+    dem = synth.fbm_16384(seed_offset=a.seed_offset, side=side)
     rng = np.random.default_rng(5)
     cover = rng.uniform(0.0, 1.0, dem.z.shape).reshape(-1)
     print(f"[longrun] generated {side}x{side} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -72,7 +72,7 @@ def main(argv=None):
             checks.append({"step": k + 1, "max_abs_dz_m": float(d.max()), "p9999_abs_dz_m": float(np.percentile(d, 99.99)),
                            "cells_above_1e-4_m": int(np.sum(d > 1e-4)), "direction_mismatches": int(np.sum(fd32 != fd64))})
             print(f"[longrun] step {k + 1}: {checks[-1]}", file=sys.stderr, flush=True)
-    out = {"workload": f"one {side}x{side} fBm DEM (config 5 generator), SFD, cover-seeded biomass",
+    out = {"workload": f"one {side}x{side} fBm DEM (5000 pits, 1000 plateaus at 16384^2), SFD, cover-seeded biomass",
            "cells": side * side, "steps": a.steps, "device_gb_before": base,
            "resident_gb": footprint,
            "ms_per_step": {n: float(np.median(v[1:] if len(v) > 1 else v)) for n, v in ms.items()},

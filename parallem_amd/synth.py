@@ -180,27 +180,32 @@ def make_dem(side: int, seed: int, pits: int, plateaus: int, outline: bool = Fal
     return Dem(z=z, rain=r, seed=seed)
 
 
-def config_dem(config: int, seed_offset: int = 0, side: int | None = None) -> Dem:
-    """The DEM of one workload configuration (sides can be shrunk for tests).
+def _scaled(n: int, s: int, base: int) -> int:
+    """A count given for a base x base DEM, scaled to the area of an s x s one."""
+    return max(1, n * s * s // (base * base))
 
-    1: 512^2 fBm, 40 pits, 10 plateaus, 5 % frame, rain 1000 mm uniform.
-    2: 4096^2 fBm, 400 pits, 100 plateaus, lognormal rain, mean 1000 mm.
-    5: 16384^2 fBm, 5000 pits, 1000 plateaus.
-    """
-    seed = BASE_SEED + 1000 * config + seed_offset
-    if config == 1:
-        s = side or 512
-        return make_dem(s, seed, pits=max(1, 40 * s * s // (512 * 512)),
-                        plateaus=max(1, 10 * s * s // (512 * 512)))
-    if config == 2:
-        s = side or 4096
-        return make_dem(s, seed, pits=max(1, 400 * s * s // (4096 * 4096)),
-                        plateaus=max(1, 100 * s * s // (4096 * 4096)), rain="lognormal")
-    if config == 5:
-        s = side or 16384
-        return make_dem(s, seed, pits=max(1, 5000 * s * s // (16384 * 16384)),
-                        plateaus=max(1, 1000 * s * s // (16384 * 16384)))
-    raise ValueError(config)
+
+def fbm_512(seed_offset: int = 0, side: int | None = None) -> Dem:
+    """512^2 fBm DEM: 40 pits, 10 plateaus, 5 % nodata frame, 1000 mm uniform
+    rain.  A smaller side (for tests) scales the pit and plateau counts."""
+    s = side or 512
+    return make_dem(s, BASE_SEED + 1000 + seed_offset, pits=_scaled(40, s, 512), plateaus=_scaled(10, s, 512))
+
+
+def fbm_4096(seed_offset: int = 0, side: int | None = None) -> Dem:
+    """4096^2 fBm DEM: 400 pits, 100 plateaus, 5 % nodata frame, lognormal
+    rain with mean 1000 mm.  A smaller side scales the counts."""
+    s = side or 4096
+    return make_dem(s, BASE_SEED + 2000 + seed_offset, pits=_scaled(400, s, 4096),
+                    plateaus=_scaled(100, s, 4096), rain="lognormal")
+
+
+def fbm_16384(seed_offset: int = 0, side: int | None = None) -> Dem:
+    """16384^2 fBm DEM: 5000 pits, 1000 plateaus, 5 % nodata frame, 1000 mm
+    uniform rain.  A smaller side scales the counts."""
+    s = side or 16384
+    return make_dem(s, BASE_SEED + 5000 + seed_offset, pits=_scaled(5000, s, 16384),
+                    plateaus=_scaled(1000, s, 16384))
 
 
 def catchment_batch(count: int, seed: int, lo: int = 256, hi: int = 2048) -> list[Dem]:

@@ -9,10 +9,10 @@ itself would take hours.  Bounds (stated, and asserted):
   <= 1e-6 over cells whose FP64 value is below 3e38 (MFD path counts can
   overflow FP32 otherwise);
 * elevation and flow accumulation over one step from the same state (the
-  FP64 trajectory's state after 50 steps of configuration 2, rounded into the
+  FP64 trajectory's state after 50 steps of the 4096^2 fBm DEM, rounded into the
   FP32 form): max |dz| <= 1e-4 m, flow accumulation max relative error
   <= 1e-6 where the directions agree;
-* the 100-step trajectories of configuration 2 (4096^2, SFD) drift apart
+* the 100-step trajectories of the 4096^2 fBm DEM (SFD) drift apart
   only through discrete flips: a direction near an exact slope tie, or a
   branch such as the bare-rock reset at soil thickness 0, taken differently
   after ~1e-7 relative differences.  Asserted: 99.99 % of cells within
@@ -52,7 +52,7 @@ def _fa_rel_error(c64, c32, it):
 
 @pytest.mark.parametrize("flowtype,side", [(0, 4096), (1, 1024)])
 def test_flow_accumulation_relative_error(flowtype, side):
-    dem = synth.config_dem(2, side=side)
+    dem = synth.fbm_4096(side=side)
     c64, c32 = _pair(dem, flowtype)
     err = _fa_rel_error(c64, c32, -1000)
     print(f"\n[fp32] flow accumulation max rel error, {side}^2 flowtype {flowtype}: {err:.3e}")
@@ -64,7 +64,7 @@ STATE = ["dem", "soilT", "Slopes", "soilM", "stone", "fines", "nut", "soilB", "T
 
 
 def test_one_step_from_the_same_state():
-    dem = synth.config_dem(2)
+    dem = synth.fbm_4096()
     c64, c32 = _pair(dem, 0)
     for k in range(50):
         c64.step(-1000 + k)
@@ -86,7 +86,7 @@ def test_one_step_from_the_same_state():
 
 
 def test_trajectory_drift_100_steps_config2():
-    dem = synth.config_dem(2)
+    dem = synth.fbm_4096()
     c64, c32 = _pair(dem, 0)
     m = c64.download("mask") == 1
     flips = []
@@ -98,7 +98,7 @@ def test_trajectory_drift_100_steps_config2():
             flips.append((k, int(np.sum(f64[m] != f32[m]))))
     d = np.abs(c64.download("dem")[m] - c32.download("dem")[m])
     worst, p9999 = float(d.max()), float(np.percentile(d, 99.99))
-    print(f"\n[fp32] config 2, 100 steps: max |dz| = {worst:.3e} m, 99.99th percentile {p9999:.3e} m, "
+    print(f"\n[fp32] 4096^2 fBm, 100 steps: max |dz| = {worst:.3e} m, 99.99th percentile {p9999:.3e} m, "
           f"cells above 1e-4 m: {int(np.sum(d > 1e-4))}; direction mismatches (step, cells): {flips}")
     assert p9999 <= 1e-4
     assert worst <= 1e-3

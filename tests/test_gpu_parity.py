@@ -58,7 +58,7 @@ def _assert_summary(ctx, orc, where):
 
 
 def test_init_state_matches_oracle():
-    dem = synth.config_dem(1, side=64)
+    dem = synth.fbm_512(side=64)
     ctx, orc = _pair(dem)
     # before the first FlowDirs the reference's fd/SFD are 0 everywhere (Data.cpp:133-155)
     _assert_fields(ctx, orc, "init", ints=("mask",),
@@ -68,7 +68,7 @@ def test_init_state_matches_oracle():
 
 @pytest.mark.parametrize("flowtype", [0, 1])
 def test_phases_bit_exact_128(flowtype):
-    dem = synth.config_dem(1, side=128, seed_offset=7)
+    dem = synth.fbm_512(side=128, seed_offset=7)
     ctx, orc = _pair(dem, flowtype)
     it = -1000
     ctx.flow_direction(it); orc.L.orc_flow_direction(orc.p, orc.s, it)
@@ -85,7 +85,7 @@ def test_phases_bit_exact_128(flowtype):
 @pytest.mark.parametrize("flowtype", [0, 1])
 def test_config1_512_ten_steps(flowtype):
     """Configuration 1 (512^2, 40 pits, 10 plateaus, 5 % frame, 1 m/yr) for 10 steps."""
-    dem = synth.config_dem(1)
+    dem = synth.fbm_512()
     ctx, orc = _pair(dem, flowtype)
     for k in range(10):
         ctx.step(-1000 + k)
@@ -98,7 +98,7 @@ def test_config1_512_ten_steps(flowtype):
 def test_reference_soilT_formula_nan_path():
     """Literal createSoilTfromformula: negative soil thickness below 400 m sends
     conc_erosion through 0/0 and NaN; the NaN pattern must match too."""
-    dem = synth.config_dem(1, side=96, seed_offset=3)
+    dem = synth.fbm_512(side=96, seed_offset=3)
     ctx, orc = _pair(dem, soilT_mode="formula")
     for k in range(3):
         ctx.step(-1000 + k)
@@ -110,7 +110,7 @@ def test_reference_soilT_formula_nan_path():
 def test_outlet_first_iteration_rule():
     """runiniter = 0 makes iteration -1000 the first: the outlet moves to the
     first cell holding the maximum flow accumulation (mfd_simple.cu:336-349)."""
-    dem = synth.config_dem(1, side=96, seed_offset=5)
+    dem = synth.fbm_512(side=96, seed_offset=5)
     ctx, orc = _pair(dem, runiniter=0)
     ctx.step(-1000)
     orc.step(-1000)
@@ -159,7 +159,7 @@ def test_temperature_change_mid_run():
     temperature field (upload) or a new climate (set_climate) between steps
     must be picked up, bit for bit with the oracle given the same change."""
     from oracle import oracle as O
-    dem = synth.config_dem(1, side=128, seed_offset=11)
+    dem = synth.fbm_512(side=128, seed_offset=11)
     ctx, orc = _pair(dem)
     it = -1000
     for k in range(2):

@@ -32,7 +32,7 @@ def _run(dem, flowtype, tuning=None, autotune=False, steps=3):
 
 @pytest.mark.parametrize("flowtype", [0, 1])
 def test_every_tuning_gives_identical_results(flowtype):
-    dem = synth.config_dem(1, side=300, seed_offset=31)
+    dem = synth.fbm_512(side=300, seed_offset=31)
     ref = _run(dem, flowtype)
     for t in CHOICES + [None]:
         got = _run(dem, flowtype, tuning=t, autotune=t is None)

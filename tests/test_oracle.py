@@ -234,7 +234,7 @@ def _minimax_bf(zz, mask, ws, nlab, W, H):
 
 
 def test_labels_follow_sfd_to_a_sink():
-    dem = synth.config_dem(1, side=48, seed_offset=2)
+    dem = synth.fbm_512(side=48, seed_offset=2)
     zz, mask, sfd, ws, nlab, raise_ = _labels_and_raise(dem.z)
     W = 48
     for i in np.nonzero(mask == 1)[0]:
@@ -254,7 +254,7 @@ def test_labels_follow_sfd_to_a_sink():
 
 def test_kruskal_spill_heights_equal_minimax():
     for seed in range(4):
-        dem = synth.config_dem(1, side=40, seed_offset=10 + seed)
+        dem = synth.fbm_512(side=40, seed_offset=10 + seed)
         zz, mask, sfd, ws, nlab, raise_ = _labels_and_raise(dem.z)
         ref = _minimax_bf(zz, mask, ws, nlab, 40, 40)
         assert np.array_equal(raise_, ref), seed
@@ -285,7 +285,7 @@ def _state_after_fd(dem, flowtype):
 @pytest.mark.parametrize("flowtype", [0, 1])
 def test_fa_topological_equals_reference_round_replay(flowtype):
     for seed in range(3):
-        o = _state_after_fd(synth.config_dem(1, side=64, seed_offset=20 + seed), flowtype)
+        o = _state_after_fd(synth.fbm_512(side=64, seed_offset=20 + seed), flowtype)
         o.L.orc_flow_accum(C.byref(o.p), C.byref(o.s), -1000)
         fa_lit = np.zeros(64 * 64)
         unc = o.L.orc_flow_accum_literal(64, 64, O.iptr(o["mask"]), O.iptr(o["fd"]), O.dptr(o["w"]), O.dptr(fa_lit))
@@ -308,7 +308,7 @@ def test_fa_planar_tilt_counts_upstream_cells():
 
 
 def test_fa_conservation_and_outlet_cycle():
-    o = _state_after_fd(synth.config_dem(1, side=80, seed_offset=4), 0)
+    o = _state_after_fd(synth.fbm_512(side=80, seed_offset=4), 0)
     o.L.orc_flow_accum(C.byref(o.p), C.byref(o.s), -1000)
     mask, fd, w, fa = o["mask"], o["fd"], o["w"], o["fa"]
     W = 80
@@ -351,7 +351,7 @@ def test_fa_mfd_path_counts():
 
 @pytest.mark.parametrize("flowtype", [0, 1])
 def test_sediment_topological_equals_reference_round_replay(flowtype):
-    dem = synth.config_dem(1, side=64, seed_offset=31)
+    dem = synth.fbm_512(side=64, seed_offset=31)
     H, W = dem.z.shape
     a = O.Oracle(dem.z, dem.rain, O.default_params(W, H, flowtype), soilT_mode="nonneg")
     a.step(-1000)                     # one full step so stone/soilT are non-trivial
@@ -427,7 +427,7 @@ def test_runoff_anchor_from_reference_dump():
 
 
 def test_step_invariants_and_clamps():
-    dem = synth.config_dem(1, side=96, seed_offset=8)
+    dem = synth.fbm_512(side=96, seed_offset=8)
     o = O.Oracle(dem.z, dem.rain, O.default_params(96, 96), soilT_mode="nonneg")
     for k in range(3):
         o.step(-1000 + k)
@@ -445,7 +445,7 @@ def test_formula_soil_thickness_goes_nan():
     """Reference init: soilT = (dem-400)/1400*5 is negative below 400 m; with
     no concentrated flow conc_erosion divides by a zero incision and
     multiplies inf by pow(0, B) = 0 (eroincidep.cu:145-149)."""
-    dem = synth.config_dem(1, side=64, seed_offset=1)
+    dem = synth.fbm_512(side=64, seed_offset=1)
     o = O.Oracle(dem.z, dem.rain, O.default_params(64, 64), soilT_mode="formula")
     o.step(-1000)
     m = o["mask"] == 1

@@ -149,7 +149,7 @@ def test_vegetation_fixed_point_and_nutrients_on_bare_rock():
 
 
 def test_outlet_drops_by_0_00024_per_step():
-    dem = synth.config_dem(1, side=64, seed_offset=21)
+    dem = synth.fbm_512(side=64, seed_offset=21)
     o = O.Oracle(dem.z, dem.rain, O.default_params(64, 64), soilT_mode="nonneg")
     out = o.s.outlet
     z0 = o["dem"][out]

@@ -1,4 +1,4 @@
-"""Synthetic workload generator: determinism and the configuration rules."""
+"""Synthetic workload generator: determinism and the rules of each workload DEM."""
 import math
 
 import numpy as np
@@ -7,13 +7,13 @@ from parallem_amd import synth
 
 
 def test_deterministic():
-    a = synth.config_dem(1, side=64)
-    b = synth.config_dem(1, side=64)
+    a = synth.fbm_512(side=64)
+    b = synth.fbm_512(side=64)
     assert np.array_equal(a.z, b.z) and np.array_equal(a.rain, b.rain)
 
 
-def test_config1_rules():
-    d = synth.config_dem(1)
+def test_fbm_512_rules():
+    d = synth.fbm_512()
     z = d.z
     assert z.shape == (512, 512)
     f = math.ceil(0.05 * 512)
@@ -27,8 +27,8 @@ def test_config1_rules():
     assert eq > 50
 
 
-def test_config2_rain_lognormal_mean():
-    d = synth.config_dem(2, side=256)
+def test_fbm_4096_rain_lognormal_mean():
+    d = synth.fbm_4096(side=256)
     m = d.z != -9999
     assert abs(d.rain[m].mean() - 1000.0) < 1e-6
     assert d.rain[m].std() > 100
