@@ -85,7 +85,7 @@ def test_one_step_from_the_same_state():
     assert rel <= 1e-6
 
 
-def test_trajectory_drift_100_steps_config2():
+def test_trajectory_drift_100_steps_fbm_4096():
     dem = synth.fbm_4096()
     c64, c32 = _pair(dem, 0)
     m = c64.download("mask") == 1

@@ -83,7 +83,7 @@ def test_phases_bit_exact_128(flowtype):
 
 
 @pytest.mark.parametrize("flowtype", [0, 1])
-def test_config1_512_ten_steps(flowtype):
+def test_fbm_512_ten_steps(flowtype):
     """The 512^2 fBm DEM (40 pits, 10 plateaus, 5 % frame, 1 m/yr rain) for 10 steps."""
     dem = synth.fbm_512()
     ctx, orc = _pair(dem, flowtype)
