@@ -57,6 +57,19 @@ typedef struct {
     int nlabels, nedges;
     int plateau_rounds1, plateau_rounds2;
     int fa_uncomputed, sed_uncomputed;
+    /* calcwater (runoffweight.cu:356-377): sum of the runoff weights over the
+     * mask cells, the "Total amount of water in grid for runoff" log line */
+    double totwater;
+    /* Input diagnostics fixed at initialisation (no reference counterpart):
+     * phantom_cells counts valid cells in rows 1..H-1 of the first
+     * P = (W/16 + 1)*16 - W columns, which the reference's label kernels
+     * (newflow.cu:358-360, 434-436) also visit from threads past the end of
+     * the previous row; when it is 0 every legal schedule of those kernels
+     * gives this library's bounds-checked labels.  soilT_negative counts valid
+     * cells whose createSoilTfromformula thickness (memory.cu:207-208) is
+     * negative (0 < dem < 400), whatever soilT_mode then does with them. */
+    int phantom_cells;
+    int soilT_negative;
 } lem_summary;
 
 /* Fields for upload/download (host layout as described above). */
@@ -77,6 +90,10 @@ typedef struct lemhip_ctx lemhip_ctx;
  * setdevicespace_* / cleardevicespace_* calls (memory_dev.cu:33-240): allocates
  * every device array once, on device `device_id`. */
 int lemhip_create(const lem_params *p, int precision, int device_id, lemhip_ctx **out);
+/* (lemhip_create and lemhip_create_atlas: when the call fails after the
+ * context was made, *out is still set so that lemhip_last_error can say why;
+ * a non-null *out must then be passed to lemhip_destroy, which frees what was
+ * allocated.) */
 /* Replaces clearDeviceSpace (memory_dev.cu:324-347). */
 int lemhip_destroy(lemhip_ctx *ctx);
 const char *lemhip_last_error(const lemhip_ctx *ctx);
@@ -178,6 +195,15 @@ int lemhip_math_eval(int fn, int n, const double *x, const double *y, double *ou
 
 /* Device properties for measurement code (no reference counterpart). */
 int lemhip_device_info(int device_id, char *name, int name_len, int *cu_count, double *mem_gb);
+/* Free and total device memory in bytes (no reference counterpart). */
+int lemhip_mem_info(int device_id, double *free_bytes, double *total_bytes);
+/* Device bytes a context allocates per grid cell in the given precision form
+ * (atlas != 0: an atlas context, which adds a slot id per cell), for
+ * admission by footprint (no reference counterpart).  The edge and label
+ * arrays of depression routing grow with the number of sinks and are not
+ * included; *grow_per_cell bounds them per cell for DEMs with at most one
+ * sink label per 16 cells. */
+int lemhip_bytes_per_cell(int precision, int atlas, double *fixed_per_cell, double *grow_per_cell);
 
 #ifdef __cplusplus
 }

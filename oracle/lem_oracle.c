@@ -521,6 +521,11 @@ void orc_runoff(const orc_params *p, orc_state *s)
         s->soilM[self] += myppt * (1. - runoff_coeff);
         if ((s->soilM[self]) > 0.5) s->soilM[self] = 0.5;
     }
+    /* calcwater, runoffweight.cu:356-377: mask cells in index order */
+    double sumwater = 0;
+    for (long i = 0; i < N; i++)
+        if (s->mask[i] == 1) sumwater += s->w[i];
+    s->totwater = sumwater;
 }
 
 /* neighbour index in direction d or -1 outside the grid */

@@ -45,6 +45,7 @@ typedef struct {
     int sinks_pass1;      /* SFD==0 mask cells after pass-1 routing */
     int plateau_rounds1, plateau_rounds2, nlabels, nedges;
     int fa_uncomputed, sed_uncomputed;
+    double totwater;      /* calcwater total of the last runoff pass */
 } orc_state;
 
 /* ---- initialisation (M2) ---- */

@@ -50,6 +50,7 @@ class OrcState(C.Structure):
         ("plateau_rounds1", C.c_int), ("plateau_rounds2", C.c_int),
         ("nlabels", C.c_int), ("nedges", C.c_int),
         ("fa_uncomputed", C.c_int), ("sed_uncomputed", C.c_int),
+        ("totwater", C.c_double),
     ]
 
 

@@ -421,6 +421,27 @@ static void launch_init_rect(lemhip_ctx *c, const DemDesc &D, int slot, const do
         c->NP);
 }
 
+// Input diagnostics of one DEM (host layout w x h): valid cells the
+// reference's label kernels also reach from threads past a row end
+// (newflow.cu:358-360: the block of 16 threads covering the last columns
+// continues into the next row's first P columns), and valid cells with a
+// negative formula soil thickness (memory.cu:207-208)
+static void input_diagnostics(const double *dem, int w, int h, int *phantom, int *soilneg)
+{
+    const int P = (w / 16 + 1) * 16 - w;
+    int ph = 0, sn = 0;
+    for (int r = 1; r < h; r++)
+        for (int q = 0; q < P && q < w; q++)
+            if (dem[(long)r * w + q] != -9999.0) ph++;
+    const long n = (long)w * h;
+    for (long i = 0; i < n; i++) {
+        const double z = dem[i];
+        if (z != -9999.0 && z > 0 && z < 400) sn++;
+    }
+    *phantom = ph;
+    *soilneg = sn;
+}
+
 // Data.cpp:38-66 (outlet), memory.cu:200-240 and Data.cpp:120-271 (initial
 // matrices) for one DEM rectangle
 static int init_slot(lemhip_ctx *c, int slot, const double *dem, const double *rain, int soilT_mode)
@@ -428,6 +449,7 @@ static int init_slot(lemhip_ctx *c, int slot, const double *dem, const double *r
     c->tcache_ok = false;   // the slot's temperatures are set
     DemDesc D;
     LEM_TRY(slot_rect(c, slot, 1, &D));
+    input_diagnostics(dem, D.w, D.h, &c->hphantom[slot], &c->hsoilneg[slot]);
     long n = (long)D.w * D.h;
     LEM_TRY(dstage_reserve(c, (rain ? 2 : 1) * n * 8));
     double *dem_d = (double *)c->dstage;
@@ -830,7 +852,6 @@ static int step_impl(lemhip_ctx *c, const int *iters, int iter)
     LEM_TRY(lem_mod_phase(c, iter));
     prof_end(c, PROF_STEP);
     LEM_CHECK(c, hipStreamSynchronize(c->s));
-    c->tune_pending = false;
     LEM_TRY(prof_resolve(c));
     return LEMHIP_OK;
 }
@@ -867,7 +888,8 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
 extern "C" int lemhip_autotune(lemhip_ctx *c)
 {
     if (!c) return LEMHIP_E_ARG;
-    c->tune_pending = true;
+    c->tune_dag_pending = true;
+    c->tune_plat_pending = true;
     return LEMHIP_OK;
 }
 
@@ -996,6 +1018,9 @@ extern "C" int lemhip_slot_summary(lemhip_ctx *c, int slot, lem_summary *o)
     o->plateau_rounds2 = c->h_rounds2;
     o->fa_uncomputed = c->h_fa_unc;
     o->sed_uncomputed = c->h_sed_unc;
+    o->totwater = h.totwater;
+    o->phantom_cells = c->hphantom[slot];
+    o->soilT_negative = c->hsoilneg[slot];
     return LEMHIP_OK;
 }
 
@@ -1042,6 +1067,39 @@ extern "C" int lemhip_math_eval(int fn, int n, const double *x, const double *y,
     }
     (void)hipFree(d);
     return rc;
+}
+
+extern "C" int lemhip_mem_info(int device_id, double *free_bytes, double *total_bytes)
+{
+    if (hipSetDevice(device_id) != hipSuccess) return LEMHIP_E_HIP;
+    size_t fr = 0, to = 0;
+    if (hipMemGetInfo(&fr, &to) != hipSuccess) return LEMHIP_E_HIP;
+    if (free_bytes) *free_bytes = (double)fr;
+    if (total_bytes) *total_bytes = (double)to;
+    return LEMHIP_OK;
+}
+
+// what alloc_ctx allocates per cell (bulk element t = 8 or 4 bytes)
+extern "C" int lemhip_bytes_per_cell(int precision, int atlas, double *fixed_per_cell, double *grow_per_cell)
+{
+    if (precision != LEMHIP_FP64 && precision != LEMHIP_FP32_KAHAN) return LEMHIP_E_ARG;
+    const double t = precision == LEMHIP_FP64 ? 8 : 4;
+    double b = 0;
+    b += 8 + 8 + 1;          // dem, soilT, mask
+    b += 8 * t;              // Slopes
+    b += 19 * t;             // soil, vegetation, climate and per-step bulk fields
+    b += 1 + 1 + 8;          // fd, SFD, fa
+    b += 4 + 4 + 4 + 4;      // plateau distances (two), candidates, labels
+    b += 2 * t;              // calcite terms
+    b += 4 + 8 + 1 + 1;      // scan / status words, raised surface, donor bits, label slots
+    if (atlas) b += 1;       // slot id
+    if (fixed_per_cell) *fixed_per_cell = b;
+    // edge lists: 64 B per slot (hash keys and values, pair and weight list,
+    // two compaction buffers), 8 slots per label rounded up to a power of
+    // two (at most 16 per label); per-label arrays 21 B with 25% headroom;
+    // for at most one sink label per 16 cells
+    if (grow_per_cell) *grow_per_cell = (16 * 64.0 + 1.25 * 21.0) / 16.0;
+    return LEMHIP_OK;
 }
 
 extern "C" int lemhip_device_info(int device_id, char *name, int name_len, int *cu_count, double *mem_gb)

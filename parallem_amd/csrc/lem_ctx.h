@@ -62,6 +62,15 @@ struct lemhip_ctx {
     DemDesc hdesc[LEM_MAX_SLOTS + 1] = {};   // active rectangles (w == 0: inactive); entry 255 stays empty
     DemDesc hrect[LEM_MAX_SLOTS] = {};       // reserved rectangles
     uint8_t hstate[LEM_MAX_SLOTS] = {};      // 0 free, 1 reserved, 2 active
+    // per-slot input diagnostics, from the host DEM at initialisation:
+    //  phantom: valid cells in rows 1..h-1 of columns < P = (w/16 + 1)*16 - w,
+    //           which the reference's label kernels also visit from threads
+    //           past the row end (newflow.cu:358-360, 434-436); 0 means every
+    //           legal schedule of those kernels equals the bounds-checked one
+    //  soilT_neg: valid cells whose formula soil thickness is negative
+    //           (0 < dem < 400, memory.cu:207-208)
+    int hphantom[LEM_MAX_SLOTS] = {};
+    int hsoilneg[LEM_MAX_SLOTS] = {};
     bool atlas = false;
     ShelfPack pack;
     DemScal *dsc = nullptr;       // device per-slot scalars (LEM_MAX_SLOTS + 1 entries)
@@ -120,7 +129,9 @@ struct lemhip_ctx {
     int dag_round = 0;
     int dag_rounds_last = 0;
     int plat_k = 8;             // plateau rounds per launch (4, 8 or 16)
-    bool tune_pending = false;  // the next step times the candidates and keeps the fastest
+    // lemhip_autotune sets both; each tuner clears its own flag once it has run
+    bool tune_dag_pending = false;    // the next flow accumulation times tile shapes and batches
+    bool tune_plat_pending = false;   // the next plateau pass on a raised surface times rounds per launch
     void *tcache = nullptr;     // MOD part 2's calcite terms per cell (2 x N, T), from the temperatures
     bool tcache_ok = false;     // false after anything that may change a temperature
     double tune_ms[16] = {};    // last tuning: per-candidate times
@@ -192,6 +203,7 @@ static inline void prof_end(lemhip_ctx *c, int id, long launches = 1)
 // phase entry points implemented in the .hip files
 int lem_fd_phase(lemhip_ctx *c, int iter);
 int lem_runoff_phase(lemhip_ctx *c);
+int lem_calcwater(lemhip_ctx *c);
 int lem_fa_phase(lemhip_ctx *c, int iter);
 int lem_mod_phase(lemhip_ctx *c, int iter);
 int lem_sync_scalars(lemhip_ctx *c);

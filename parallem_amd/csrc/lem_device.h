@@ -55,6 +55,7 @@ struct DemScal {
     double base_level;    // dem[outlet] - 0.00012 (updates.cu:151)
     double FA_max;
     double totE, totI, totG, totD, totwC, totwP, totbio;
+    double totwater;      // sum of runoff weights over mask cells (runoffweight.cu:356-377)
     unsigned long long lowest_enc, famax_enc;
     int outlet;           // atlas index of the outlet cell
     int argidx;

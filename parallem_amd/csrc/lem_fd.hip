@@ -1472,7 +1472,7 @@ static int fd_phase_t(lemhip_ctx *c, int iter)
         k_slots_sinkcount_fin<<<(c->nslot + 255) / 256, 256, 0, c->s>>>(c->ddesc, c->nslot, c->dsc);
         prof_end(c, PROF_FLOOD);
         int n2 = 0, r2 = 0;
-        if (c->tune_pending) LEM_TRY(tune_plateau<TS>(c, c->z2, (TS *)nullptr));
+        if (c->tune_plat_pending) LEM_TRY(tune_plateau<TS>(c, c->z2, (TS *)nullptr));
         LEM_TRY(directions_and_plateaus<TS>(c, c->z2, (TS *)nullptr, &n2, &r2));
         c->h_rounds2 = r2;
     }

@@ -26,8 +26,13 @@ __device__ __forceinline__ float mtanh(float x) { return tanhf(x); }
 template <typename T> struct FaS;
 template <> struct FaS<double> { using type = double; };
 template <> struct FaS<float> { using type = float2; };
-__device__ __forceinline__ double fa_val(double v) { return v; }
-__device__ __forceinline__ float fa_val(float2 v) { return v.x; }
+// flow accumulation as one double: the FP32 form's float pair (hi, lo) holds
+// ~48 significant bits, and every quantity MOD derives from it (overland flow
+// and its cap, the incision threshold, the gelifluction cut-off, surface flow
+// against evapotranspiration) is formed from the full pair in double, so a
+// threshold decides the same branch as in the FP64 form
+__device__ __forceinline__ double fa_full(double v) { return v; }
+__device__ __forceinline__ double fa_full(float2 v) { return (double)v.x + (double)v.y; }
 __device__ __forceinline__ void fa_set(double *p, double v) { *p = v; }
 __device__ __forceinline__ void fa_set(float2 *p, float v) { *p = make_float2(v, 0.f); }
 
@@ -80,7 +85,7 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
                                               T *__restrict__ slopeptr, T *__restrict__ ero, T *__restrict__ inci,
                                               T *__restrict__ geli, T *__restrict__ depo)
 {
-    const T rescale_c = (T)(1 / (10000 / (p.cellsize * p.cellsize)));
+    const double rescale_c = 1 / (10000 / (p.cellsize * p.cellsize));
     for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
         if (mask[self] == 0) {
             slopeptr[self] = 0; ero[self] = 0; inci[self] = 0; geli[self] = 0; depo[self] = 0;
@@ -98,14 +103,15 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         if ((total > 0) && (nonzero > 0)) sp = total / (T)nonzero;
         slopeptr[self] = sp;
         T sT = (T)soilT[self];
-        T fav = (T)fa_val(fa[self]);
+        const double fad = fa_full(fa[self]);
         T ex = mexp((T)-0.07 * (TotB[self]) / (T)76.5);
         // diffuse
         T erodibility = ((T)2.77e-6 * mpow(fines[self], (T)1.14) * ((T)12. - nut[self]))
                         + ((T)0.043 * ((T)p.soil_struct - (T)2.)) + ((T)0.033 * ((T)p.profile_permeability - (T)3.));
         if (erodibility < (T)0.0) erodibility = 0;
-        T OF = ((fav) * (T)1000) / (T)25;
-        if (OF > (T)p.Qthreshold) OF = (T)p.Qthreshold;
+        double OFd = ((fad) * 1000) / 25;
+        if (OFd > p.Qthreshold) OFd = p.Qthreshold;
+        const T OF = (T)OFd;
         T slope = sp;
         if (slope == (T)0.0) slope = (T)0.0001;
         if (slope > (T)5.0) slope = (T)5.0;
@@ -117,9 +123,10 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         T cslope = sp;
         if (cslope > (T)0.5) cslope = (T)0.5;
         T max_incision = (T)p.max_incision;
-        T f_temp = ((fav) * rescale_c) - (T)p.Qthreshold;
+        const double ftd = ((fad) * rescale_c) - p.Qthreshold;
+        T f_temp = (T)ftd;
         T iv;
-        if (f_temp < (T)0.0) { f_temp = 0; iv = 0; }
+        if (ftd < 0.0) { f_temp = 0; iv = 0; }
         else {
             if (cslope == (T)0.0) iv = (T)p.Flow_A * mpow(f_temp, (T)p.Flow_B) * (T)0.01 * ex;
             else iv = (T)p.Flow_A * mpow(f_temp, (T)p.Flow_B) * (cslope) * ex;
@@ -138,8 +145,9 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         if (tc >= (T)-20. && tc <= (T)8.87)
             kappa = (T)0.15711 - (T)0.011835 * tc - (T)0.001195 * tc * tc + (T)0.000038 * tc * tc * tc
                     + (T)0.000002 * tc * tc * tc * tc;
-        T scaler = (T)1 - (fav / (T)p.Gflowlim);
-        if (fav > (T)p.Gflowlim) scaler = (T)0.0;
+        double scd = 1 - (fad / p.Gflowlim);
+        if (fad > p.Gflowlim) scd = 0.0;
+        const T scaler = (T)scd;
         T slopethreshold = sp;
         if (slopethreshold > (T)p.Gslope) slopethreshold = (T)p.Gslope;
         T g = kappa * slopethreshold * scaler;
@@ -222,17 +230,23 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
         dzo[self] = dz;
         // weathering
         T rn = rain[self];
-        T PE_d = (T)1.44 * mpow(tc + (T)6., (T)2);
+        // surface flow and evapotranspiration in double in both forms (FP64:
+        // the reference's own expressions; FP32: the bulk inputs widened)
+        double PE_d;
+        if (sizeof(T) == 8) PE_d = (double)((T)1.44 * mpow(tc + (T)6., (T)2));
+        else PE_d = 1.44 * (((double)tc + 6.) * ((double)tc + 6.));
         T P0_d = (T)7.7e-5, P1_d = (T)2.3;
-        T surface_flow = (T)fa_val(fa[self]) * (T)1000;
-        T evap_trans;
-        if (rn + surface_flow <= PE_d) evap_trans = rn + surface_flow;
-        else evap_trans = rn + surface_flow - PE_d;
+        const double surface_flow = fa_full(fa[self]) * 1000;
+        const double rnd = (double)rn;
+        double evap_trans;
+        if (rnd + surface_flow <= PE_d) evap_trans = rnd + surface_flow;
+        else evap_trans = rnd + surface_flow - PE_d;
         double sT = soilT[self];
         T Ca2eq = (sT > 0.1) ? ca_hi[self] : ca_lo[self];
-        T wC = (T)0.0148148148 * Ca2eq * (surface_flow / (T)rescale_w);
-        if (wC <= (T)0.0) wC = 0;
-        if (wC > (T)0.001) wC = (T)0.001;
+        double wCd = 0.0148148148 * (double)Ca2eq * (surface_flow / rescale_w);
+        if (wCd <= 0.0) wCd = 0;
+        if (wCd > 0.001) wCd = 0.001;
+        const T wC = (T)wCd;
         T wP = P0_d * mexp(-(P1_d) * (T)sT);
         if (sizeof(T) == 8) wP = (T)(7.7e-5 * lm_exp(-(2.3) * sT));
         wCo[self] = wC;
@@ -248,7 +262,7 @@ __global__ void __launch_bounds__(256) k_mod2(long N, lem_params p, const uint8_
             if (st < (T)0.01) st = (T)0.01;
             else if (st > (T)99.9) st = (T)99.9;
             fi = (T)100. - st;
-            sm -= (T)((double)((T)1.e-3 * evap_trans) / sT);
+            sm -= (T)((1.e-3 * evap_trans) / sT);
             if (sm < (T)0.02) sm = (T)0.02;
             else if (sm > (T)0.5) sm = (T)0.5;
         }
@@ -322,11 +336,12 @@ __device__ __forceinline__ double block_sum_f64(double v, double *sh)
     return r;
 }
 
-// up to three fields summed in one pass; partial[(j*3 + q)*SLOT_KB + k]
+// up to three fields summed in one pass; partial[(j*3 + q)*SLOT_KB + k];
+// with a mask, cells outside it are skipped
 template <typename T>
 __global__ void __launch_bounds__(256) k_slot_sum3(const T *__restrict__ a0, const T *__restrict__ a1,
-                                                   const T *__restrict__ a2, int W, const DemDesc *dd,
-                                                   double *partial)
+                                                   const T *__restrict__ a2, const uint8_t *__restrict__ mask,
+                                                   int W, const DemDesc *dd, double *partial)
 {
     __shared__ double sh[256];
     int j = blockIdx.y;
@@ -334,6 +349,7 @@ __global__ void __launch_bounds__(256) k_slot_sum3(const T *__restrict__ a0, con
     if (slot_block_idle(D)) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     slot_rows(D, W, [&](long i) {
+        if (mask && !mask[i]) return;
         s0 += (double)a0[i];
         if (a1) s1 += (double)a1[i];
         if (a2) s2 += (double)a2[i];
@@ -371,11 +387,11 @@ __global__ void __launch_bounds__(256) k_slot_sum_final(const double *partial, i
 
 template <typename T>
 static int slot_sum(lemhip_ctx *c, const void *a0, int off0, const void *a1 = nullptr, int off1 = 0,
-                    const void *a2 = nullptr, int off2 = 0)
+                    const void *a2 = nullptr, int off2 = 0, const uint8_t *mask = nullptr)
 {
     dim3 g(c->slot_kbmax, c->nslot);
     k_slot_sum3<T><<<g, 256, 0, c->s>>>(reinterpret_cast<const T *>(a0), reinterpret_cast<const T *>(a1),
-                                        reinterpret_cast<const T *>(a2), c->W, c->ddesc, c->partials);
+                                        reinterpret_cast<const T *>(a2), mask, c->W, c->ddesc, c->partials);
     int na = a2 ? 3 : (a1 ? 2 : 1);
     k_slot_sum_final<<<c->nslot, 256, 0, c->s>>>(c->partials, na, c->ddesc, c->dsc, off0, off1, off2);
     LEM_LAUNCHED(c);
@@ -477,9 +493,6 @@ int lem_slot_first_lowest(lemhip_ctx *c, const double *z, int s0, int ns)
     return LEMHIP_OK;
 }
 
-__device__ __forceinline__ double fa_full(double v) { return v; }
-__device__ __forceinline__ double fa_full(float2 v) { return (double)v.x + (double)v.y; }
-
 // max of fa with the reference's start value 0 (NaN never wins)
 template <typename FS>
 __global__ void __launch_bounds__(256) k_slot_famax(const FS *__restrict__ fa, int W, const DemDesc *dd, DemScal *ds)
@@ -568,18 +581,33 @@ static int runoff_t(lemhip_ctx *c)
     return LEMHIP_OK;
 }
 
-int lem_runoff_phase(lemhip_ctx *c)
+#define DS_OFF(f) ((int)offsetof(DemScal, f))
+
+// calcwater (runoffweight.cu:356-377): total runoff weight over mask cells
+template <typename T>
+static int calcwater_t(lemhip_ctx *c)
 {
-    return c->prec == LEMHIP_FP64 ? runoff_t<double>(c) : runoff_t<float>(c);
+    return slot_sum<T>(c, c->w, DS_OFF(totwater), nullptr, 0, nullptr, 0, c->mask);
 }
 
-#define DS_OFF(f) ((int)offsetof(DemScal, f))
+int lem_calcwater(lemhip_ctx *c)
+{
+    return c->prec == LEMHIP_FP64 ? calcwater_t<double>(c) : calcwater_t<float>(c);
+}
+
+// computeRunOffWeights then calcwater (lem.cpp:88-89)
+int lem_runoff_phase(lemhip_ctx *c)
+{
+    LEM_TRY(c->prec == LEMHIP_FP64 ? runoff_t<double>(c) : runoff_t<float>(c));
+    return lem_calcwater(c);
+}
+
 
 template <typename T>
 static int fa_t(lemhip_ctx *c)
 {
     using FS = typename FaS<T>::type;
-    if (c->tune_pending) LEM_TRY(lem_tune_dag(c));
+    if (c->tune_dag_pending) LEM_TRY(lem_tune_dag(c));
     prof_begin(c, PROF_FA);
     LEM_TRY(lem_fa_dag<T>(c));
     prof_end(c, PROF_FA, c->dag_rounds_last);
@@ -622,7 +650,10 @@ static int mod_t(lemhip_ctx *c)
     prof_end(c, PROF_SED, c->dag_rounds_last);
     LEM_TRY(slot_sum<T>(c, c->depo, DS_OFF(totD)));
     k_slots_base<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
-    T *ca_hi = reinterpret_cast<T *>(c->tcache), *ca_lo = ca_hi + N;
+    // the low-soil variant sits one allocated grid after the other, whatever
+    // band of rows this step covers (an atlas steps only its used rows, and
+    // that band grows and shrinks as DEMs come and go)
+    T *ca_hi = reinterpret_cast<T *>(c->tcache), *ca_lo = ca_hi + c->NP;
     if (!c->tcache_ok) {
         k_calcite_terms<T><<<g, 256, 0, c->s>>>(N, c->mask, (const T *)c->temp, ca_hi, ca_lo);
         LEM_LAUNCHED(c);

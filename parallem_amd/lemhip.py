@@ -42,6 +42,7 @@ EXPORTS = [
     "lemhip_slot_init", "lemhip_slot_close", "lemhip_slot_rect", "lemhip_slot_upload", "lemhip_slot_download",
     "lemhip_slot_get_outlet", "lemhip_slot_set_outlet", "lemhip_slot_set_climate", "lemhip_slot_summary",
     "lemhip_step_slots", "lemhip_get_tuning", "lemhip_set_tuning", "lemhip_autotune",
+    "lemhip_mem_info", "lemhip_bytes_per_cell",
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
@@ -85,6 +86,7 @@ class LemSummary(C.Structure):
         ("outletcellidx", C.c_int), ("sinkcount", C.c_int), ("sinks_pass1", C.c_int),
         ("nlabels", C.c_int), ("nedges", C.c_int), ("plateau_rounds1", C.c_int),
         ("plateau_rounds2", C.c_int), ("fa_uncomputed", C.c_int), ("sed_uncomputed", C.c_int),
+        ("totwater", C.c_double), ("phantom_cells", C.c_int), ("soilT_negative", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -151,6 +153,9 @@ def lib() -> C.CDLL:
     L.lemhip_get_tuning.argtypes = [vp, C.POINTER(LemTuning)]
     L.lemhip_set_tuning.argtypes = [vp, C.POINTER(LemTuning)]
     L.lemhip_autotune.argtypes = [vp]
+    dp = C.POINTER(C.c_double)
+    L.lemhip_mem_info.argtypes = [C.c_int, dp, dp]
+    L.lemhip_bytes_per_cell.argtypes = [C.c_int, C.c_int, dp, dp]
     _lib = L
     return L
 
@@ -417,6 +422,25 @@ def math_eval(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray
     if rc != 0:
         raise LemError(f"lemhip_math_eval: {rc}")
     return out
+
+
+def mem_info(dev: int = 0) -> tuple[float, float]:
+    """(free, total) device memory in bytes."""
+    fr, to = C.c_double(), C.c_double()
+    rc = lib().lemhip_mem_info(dev, C.byref(fr), C.byref(to))
+    if rc != 0:
+        raise LemError("lemhip_mem_info failed")
+    return fr.value, to.value
+
+
+def bytes_per_cell(precision: int, atlas: bool = True) -> tuple[float, float]:
+    """(fixed, growth bound) device bytes per cell of a context
+    (lemhip_bytes_per_cell)."""
+    fx, gr = C.c_double(), C.c_double()
+    rc = lib().lemhip_bytes_per_cell(precision, 1 if atlas else 0, C.byref(fx), C.byref(gr))
+    if rc != 0:
+        raise LemError("lemhip_bytes_per_cell failed")
+    return fx.value, gr.value
 
 
 def device_info(dev: int = 0) -> dict:

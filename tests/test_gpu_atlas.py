@@ -115,3 +115,33 @@ def test_atlas_room_and_state_errors():
     assert atlas.L.lemhip_summary(atlas.h, ctypes.byref(summ)) == -5      # single-DEM call on an atlas
     assert atlas.L.lemhip_slot_summary(atlas.h, 200, ctypes.byref(summ)) == -1
     atlas.close()
+
+
+@pytest.mark.parametrize("precision", [lemhip.FP64, lemhip.FP32_KAHAN])
+def test_closing_top_shelf_keeps_other_slots(precision):
+    """Closing the DEM on the top shelf shrinks the band of rows a step
+    covers.  The other DEM (low ground: soil thickness 0, so weathering takes
+    the thin-soil calcite term) must keep stepping exactly as alone, with no
+    initialisation or climate call in between to refresh cached terms."""
+    low = synth.make_dem(96, 21, pits=3, plateaus=2, height=70)
+    top = synth.make_dem(120, 22, pits=2, plateaus=1, height=150)
+
+    def params(W, H):
+        return lemhip.default_params(W, H, 0)
+
+    atlas = lemhip.LemAtlas(params(200, 300), 200, 300, precision, 0)
+    sa = atlas.add(low.z, low.rain, soilT_mode="nonneg")
+    sb = atlas.add(top.z, top.rain, soilT_mode="nonneg")
+    assert atlas.rect(sa)[1] == 0 and atlas.rect(sb)[1] == 97   # separate shelves, sb on top
+    alone = Alone(low, params, precision)
+    for k in range(2):
+        atlas.step()
+        alone.step()
+    _compare(atlas, sa, alone, f"before close, step {k}")
+    atlas.remove(sb)
+    for k in range(2, 5):
+        atlas.step()
+        alone.step()
+        _compare(atlas, sa, alone, f"after closing the top shelf, step {k}")
+    assert alone.ctx.summary()["totweatherC"] > 0
+    atlas.close()

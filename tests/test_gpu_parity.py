@@ -51,7 +51,7 @@ def _assert_summary(ctx, orc, where):
     for k in ("sinkcount", "fa_uncomputed", "sed_uncomputed"):
         assert s[k] == o[k], f"{where}: {k} {s[k]} vs {o[k]}"
     assert s["FA_max"] == o["FA_max"], f"{where}: FA_max"
-    for k, ok in (("totE", "totE"), ("totI", "totI"), ("totG", "totG"), ("totD", "totD"),
+    for k, ok in (("totwater", "totwater"), ("totE", "totE"), ("totI", "totI"), ("totG", "totG"), ("totD", "totD"),
                   ("totweatherC", "totwC"), ("totweatherP", "totwP"), ("totbio", "totbio")):
         a, b = s[k], o[ok]
         assert abs(a - b) <= 1e-9 * max(1.0, abs(b)), f"{where}: {k} {a} vs {b}"
@@ -109,13 +109,47 @@ def test_reference_soilT_formula_nan_path():
 
 def test_outlet_first_iteration_rule():
     """runiniter = 0 makes iteration -1000 the first: the outlet moves to the
-    first cell holding the maximum flow accumulation (mfd_simple.cu:336-349)."""
+    first cell holding the maximum flow accumulation (mfd_simple.cu:336-349),
+    and the surface update then lifts every cell at or below the new base
+    level (updates.cu:151-161)."""
     dem = synth.fbm_512(side=96, seed_offset=5)
     ctx, orc = _pair(dem, runiniter=0)
+    o_init = orc.s.outlet
+    for k in range(2):
+        ctx.step(-1000 + k)
+        orc.step(-1000 + k)
+        assert ctx.outlet == orc.s.outlet
+        _assert_fields(ctx, orc, f"runin0 step {k}")
+        _assert_summary(ctx, orc, f"runin0 step {k}")
+    assert orc.s.outlet != o_init
+
+
+def test_outlet_check_lowers_a_raised_outlet():
+    """The outlet check (routeflow.cu:171-184, 739-741): when the outlet is
+    not the lowest valid cell at the start of a step, it is set to
+    lowest - 1e-4 after FlowDirs has read the old surface and before flow
+    accumulation.  A step always ends with the outlet as the strict minimum
+    (the surface update lifts every cell at or below the base level), so the
+    branch fires only after the caller moves the outlet or edits the surface;
+    here the outlet is moved to a higher cell between steps."""
+    dem = synth.fbm_512(side=96, seed_offset=5)
+    ctx, orc = _pair(dem)
     ctx.step(-1000)
     orc.step(-1000)
-    assert ctx.outlet == orc.s.outlet
-    _assert_fields(ctx, orc, "runin0")
+    valid = np.flatnonzero(orc["mask"] == 1)
+    for k, pick in ((1, valid[len(valid) // 3]), (2, valid[2 * len(valid) // 3])):
+        ctx.outlet = int(pick)
+        orc.s.outlet = int(pick)
+        z = orc["dem"].copy()
+        lowest = min(1000.0, float(z[valid].min()))
+        assert lowest < z[pick]
+        ctx.step(-1000 + k)
+        orc.step(-1000 + k)
+        _assert_fields(ctx, orc, f"moved outlet, step {k}")
+        _assert_summary(ctx, orc, f"moved outlet, step {k}")
+        # lowered to lowest - 1e-4, then the surface update pinned it
+        # 0.00012 below the base level it defined (updates.cu:151,165-174)
+        assert orc["dem"][pick] == ((lowest - 1e-4) - 0.00012) - 0.00012
 
 
 def test_no_nodata_frame_and_tiny_grids():
