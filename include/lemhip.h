@@ -19,6 +19,8 @@
 #ifndef LEMHIP_H
 #define LEMHIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -156,6 +158,21 @@ int lemhip_slot_set_outlet(lemhip_ctx *ctx, int slot, int outlet);
 int lemhip_slot_set_climate(lemhip_ctx *ctx, int slot, double rain, double temp);
 int lemhip_slot_summary(lemhip_ctx *ctx, int slot, lem_summary *out);
 int lemhip_step_slots(lemhip_ctx *ctx, const int *iters);
+
+/* Checkpoint / restart of one DEM (slot 0 of a single-DEM context, or an
+ * atlas slot): replaces retriveProcessMatrices (ioGDAL.cpp:662-804), which
+ * reads the process matrices of a restart back from grids written by
+ * savegrids (ioGDAL.cpp:208-324).  The snapshot is a self-describing byte
+ * buffer: dem, soilT, mask, the sticky Slopes (which the reference's restart
+ * leaves out), soilM, stone, fines, nut, soilB, TotB, the four climate
+ * fields and the outlet, in the context's own element types.  Restoring it
+ * into a context of the same precision form and DEM size (an open slot, or a
+ * single-DEM context straight after lemhip_create) reproduces the state
+ * exactly, so the resumed run is bit-identical to an unbroken one.
+ * snapshot_size needs an open slot; snapshot needs an initialised one. */
+int lemhip_snapshot_size(lemhip_ctx *ctx, int slot, size_t *bytes);
+int lemhip_snapshot(lemhip_ctx *ctx, int slot, void *buf, size_t bytes);
+int lemhip_restore(lemhip_ctx *ctx, int slot, const void *buf, size_t bytes);
 
 /* Launch tuning (no reference counterpart).  The dependency engine of the
  * two accumulation passes runs in tiles of dag_tile_w x dag_tile_h cells

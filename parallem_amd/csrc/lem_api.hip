@@ -1031,6 +1031,132 @@ extern "C" int lemhip_summary(lemhip_ctx *c, lem_summary *o)
     return lemhip_slot_summary(c, 0, o);
 }
 
+// ------------------------------------------------------ checkpoint / restart
+
+// A snapshot holds what carries one DEM from a step to the next: the
+// surface, soil thickness, mask, the sticky Slopes (the reference's restart,
+// retriveProcessMatrices ioGDAL.cpp:662-804, reads height, stone, fines,
+// soilM, soilB, soilT, nut and totbio back but not Slopes, so its resumed
+// run's aveslope differs from an unbroken one), the soil, vegetation and
+// climate fields and the outlet.  Fields are in the DEM's own row-major
+// layout with the context's element types (bulk fields T, Slopes as 8
+// planes of T, mask one byte), so restoring into a context of the same
+// precision form is exact.
+struct SnapHeader {
+    char magic[8];
+    int version, precision, w, h, outlet, nfields;
+    double cellsize;
+    long long bytes;
+};
+static const int kSnapFields[] = {LEM_F_DEM,   LEM_F_SOILT, LEM_F_MASK,      LEM_F_SLOPES,   LEM_F_SOILM,
+                                  LEM_F_STONE, LEM_F_FINES, LEM_F_NUT,       LEM_F_SOILB,    LEM_F_TOTB,
+                                  LEM_F_RAIN,  LEM_F_TEMP,  LEM_F_LAST_RAIN, LEM_F_LAST_TEMP};
+static const int kSnapN = (int)(sizeof(kSnapFields) / sizeof(kSnapFields[0]));
+
+static size_t snap_bytes(const lemhip_ctx *c, const DemDesc &D)
+{
+    size_t n = (size_t)D.w * D.h, b = sizeof(SnapHeader);
+    for (int q = 0; q < kSnapN; q++) {
+        int kind = -1;
+        (void)field_ptr(const_cast<lemhip_ctx *>(c), kSnapFields[q], &kind);
+        b += (kind == 4 ? 8 : 1) * n * kind_elem(c, kind);
+    }
+    return b;
+}
+
+static int snap_slot(lemhip_ctx *c, int slot, int need, DemDesc *D)
+{
+    if (!c->atlas && slot != 0) { c->err = "single-DEM context: slot must be 0"; return LEMHIP_E_ARG; }
+    return slot_rect(c, slot, need, D);
+}
+
+extern "C" int lemhip_snapshot_size(lemhip_ctx *c, int slot, size_t *bytes)
+{
+    if (!c || !bytes) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(snap_slot(c, slot, 1, &D));
+    *bytes = snap_bytes(c, D);
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_snapshot(lemhip_ctx *c, int slot, void *buf, size_t bytes)
+{
+    if (!c || !buf) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(snap_slot(c, slot, 2, &D));
+    const size_t need = snap_bytes(c, D);
+    if (bytes < need) { c->err = "snapshot buffer too small"; return LEMHIP_E_ARG; }
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    SnapHeader h = {};
+    memcpy(h.magic, "LEMSNAP", 8);
+    h.version = 1;
+    h.precision = c->prec;
+    h.w = D.w;
+    h.h = D.h;
+    LEM_TRY(lem_read_slots(c));
+    const int o = c->hdsc[slot].outlet;
+    h.outlet = (o / c->W - D.y0) * D.w + (o % c->W - D.x0);
+    h.nfields = kSnapN;
+    h.cellsize = c->p.cellsize;
+    h.bytes = (long long)need;
+    memcpy(buf, &h, sizeof h);
+    char *out = (char *)buf + sizeof h;
+    const size_t n = (size_t)D.w * D.h;
+    for (int q = 0; q < kSnapN; q++) {
+        int kind = -1;
+        void *src = field_ptr(c, kSnapFields[q], &kind);
+        const size_t es = kind_elem(c, kind);
+        const int planes = kind == 4 ? 8 : 1;
+        LEM_TRY(stage_reserve(c, planes * n * es));
+        for (int pl = 0; pl < planes; pl++) LEM_TRY(rect_copy(c, c->stage, src, pl, es, D, true));
+        LEM_CHECK(c, hipStreamSynchronize(c->s));
+        memcpy(out, c->stage, planes * n * es);
+        out += planes * n * es;
+    }
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_restore(lemhip_ctx *c, int slot, const void *buf, size_t bytes)
+{
+    if (!c || !buf || bytes < sizeof(SnapHeader)) return LEMHIP_E_ARG;
+    DemDesc D;
+    LEM_TRY(snap_slot(c, slot, 1, &D));
+    SnapHeader h;
+    memcpy(&h, buf, sizeof h);
+    if (memcmp(h.magic, "LEMSNAP", 8) != 0 || h.version != 1) { c->err = "not a snapshot"; return LEMHIP_E_ARG; }
+    if (h.precision != c->prec) { c->err = "snapshot of the other precision form"; return LEMHIP_E_ARG; }
+    if (h.w != D.w || h.h != D.h) { c->err = "snapshot of a DEM of another size"; return LEMHIP_E_ARG; }
+    if (h.nfields != kSnapN || (size_t)h.bytes != snap_bytes(c, D) || bytes < (size_t)h.bytes) {
+        c->err = "snapshot size mismatch";
+        return LEMHIP_E_ARG;
+    }
+    if (h.outlet < 0 || h.outlet >= D.w * D.h) { c->err = "snapshot outlet out of range"; return LEMHIP_E_ARG; }
+    LEM_CHECK(c, hipSetDevice(c->dev));
+    const size_t n = (size_t)D.w * D.h;
+    // the rectangle starts from the snapshot's surface (slot id, defaults of
+    // the per-step fields, per-slot scalars), then every field is replaced
+    std::vector<double> dem(n);
+    memcpy(dem.data(), (const char *)buf + sizeof h, n * 8);
+    LEM_TRY(init_slot(c, slot, dem.data(), nullptr, 0));
+    const char *in = (const char *)buf + sizeof h;
+    for (int q = 0; q < kSnapN; q++) {
+        int kind = -1;
+        void *dst = field_ptr(c, kSnapFields[q], &kind);
+        const size_t es = kind_elem(c, kind);
+        const int planes = kind == 4 ? 8 : 1;
+        LEM_TRY(stage_reserve(c, planes * n * es));
+        memcpy(c->stage, in, planes * n * es);
+        for (int pl = 0; pl < planes; pl++) LEM_TRY(rect_copy(c, c->stage, dst, pl, es, D, false));
+        LEM_CHECK(c, hipStreamSynchronize(c->s));
+        in += planes * n * es;
+    }
+    c->tcache_ok = false;
+    c->hint[8] = (D.y0 + h.outlet / D.w) * c->W + D.x0 + h.outlet % D.w;
+    LEM_CHECK(c, hipMemcpyAsync(&c->dsc[slot].outlet, &c->hint[8], sizeof(int), hipMemcpyHostToDevice, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    return LEMHIP_OK;
+}
+
 // ------------------------------------------------------------ test hooks
 
 __global__ void k_math_eval(int fn, int n, const double *x, const double *y, double *out)

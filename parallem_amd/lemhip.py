@@ -42,7 +42,7 @@ EXPORTS = [
     "lemhip_slot_init", "lemhip_slot_close", "lemhip_slot_rect", "lemhip_slot_upload", "lemhip_slot_download",
     "lemhip_slot_get_outlet", "lemhip_slot_set_outlet", "lemhip_slot_set_climate", "lemhip_slot_summary",
     "lemhip_step_slots", "lemhip_get_tuning", "lemhip_set_tuning", "lemhip_autotune",
-    "lemhip_mem_info", "lemhip_bytes_per_cell",
+    "lemhip_mem_info", "lemhip_bytes_per_cell", "lemhip_snapshot_size", "lemhip_snapshot", "lemhip_restore",
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
@@ -156,6 +156,9 @@ def lib() -> C.CDLL:
     dp = C.POINTER(C.c_double)
     L.lemhip_mem_info.argtypes = [C.c_int, dp, dp]
     L.lemhip_bytes_per_cell.argtypes = [C.c_int, C.c_int, dp, dp]
+    L.lemhip_snapshot_size.argtypes = [vp, C.c_int, C.POINTER(C.c_size_t)]
+    L.lemhip_snapshot.argtypes = [vp, C.c_int, vp, C.c_size_t]
+    L.lemhip_restore.argtypes = [vp, C.c_int, vp, C.c_size_t]
     _lib = L
     return L
 
@@ -233,6 +236,17 @@ class _Handle:
             t.plateau_rounds_per_launch = plateau_rounds_per_launch
         self._check(self.L.lemhip_set_tuning(self.h, C.byref(t)), "set_tuning")
 
+    def _snapshot(self, slot: int) -> bytes:
+        n = C.c_size_t()
+        self._check(self.L.lemhip_snapshot_size(self.h, slot, C.byref(n)), "snapshot_size")
+        buf = np.empty(n.value, np.uint8)
+        self._check(self.L.lemhip_snapshot(self.h, slot, _ptr(buf), n.value), "snapshot")
+        return buf.tobytes()
+
+    def _restore(self, slot: int, snap: bytes) -> None:
+        buf = np.frombuffer(snap, np.uint8).copy()
+        self._check(self.L.lemhip_restore(self.h, slot, _ptr(buf), buf.size), "restore")
+
     def autotune(self) -> None:
         """The next step times every launch candidate on this context's data
         and keeps the fastest (results do not depend on the choice)."""
@@ -306,6 +320,14 @@ class LemContext(_Handle):
         s = LemSummary()
         self._check(self.L.lemhip_summary(self.h, C.byref(s)), "summary")
         return s.as_dict()
+
+    def snapshot(self) -> bytes:
+        """Checkpoint of the resident state (lemhip_snapshot)."""
+        return self._snapshot(0)
+
+    def restore(self, snap: bytes) -> None:
+        """Resume from a checkpoint of a DEM of this size and precision form."""
+        self._restore(0, snap)
 
 
 class LemAtlas(_Handle):
@@ -395,6 +417,14 @@ class LemAtlas(_Handle):
         s = LemSummary()
         self._check(self.L.lemhip_slot_summary(self.h, slot, C.byref(s)), "slot_summary")
         return s.as_dict()
+
+    def snapshot(self, slot: int) -> bytes:
+        return self._snapshot(slot)
+
+    def restore(self, slot: int, snap: bytes, iteration: int) -> None:
+        """Resume an open slot from a checkpoint; its next step is `iteration`."""
+        self._restore(slot, snap)
+        self.iters[slot] = iteration
 
 
 # reference-named entry points (lem.cpp:84-96)

@@ -123,8 +123,8 @@ def test_closing_top_shelf_keeps_other_slots(precision):
     covers.  The other DEM (low ground: soil thickness 0, so weathering takes
     the thin-soil calcite term) must keep stepping exactly as alone, with no
     initialisation or climate call in between to refresh cached terms."""
-    low = synth.make_dem(96, 21, pits=3, plateaus=2, height=70)
-    top = synth.make_dem(120, 22, pits=2, plateaus=1, height=150)
+    low = synth.make_dem(96, 21, pits=3, plateaus=2, height=70)      # 96 x 70 cells
+    top = synth.make_dem(120, 22, pits=2, plateaus=1, height=150)    # too wide for the first shelf's rest
 
     def params(W, H):
         return lemhip.default_params(W, H, 0)
@@ -132,7 +132,9 @@ def test_closing_top_shelf_keeps_other_slots(precision):
     atlas = lemhip.LemAtlas(params(200, 300), 200, 300, precision, 0)
     sa = atlas.add(low.z, low.rain, soilT_mode="nonneg")
     sb = atlas.add(top.z, top.rain, soilT_mode="nonneg")
-    assert atlas.rect(sa)[1] == 0 and atlas.rect(sb)[1] == 97   # separate shelves, sb on top
+    assert atlas.rect(sa)[1] == 0 and atlas.rect(sb)[1] == 71   # separate shelves, sb on top
+    valid = low.z != -9999.0
+    assert np.mean(low.z[valid] < 428.0) > 0.2                   # thin-soil cells (formula soil <= 0.1 m)
     alone = Alone(low, params, precision)
     for k in range(2):
         atlas.step()
