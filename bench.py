@@ -187,16 +187,34 @@ def measured_traffic(kernel, tuning, batch):
     return None
 
 
-def cpu_baseline(side, threads):
+def cpu_step(side, threads):
+    """One FP64 timestep of the C restatement on a side^2 DEM from the 4096^2
+    generator: cells/s and seconds.  threads 0: the serial build."""
     from oracle import oracle as O
     from parallem_amd import synth
     dem = synth.fbm_4096(side=side)
-    O.lib(True).orc_set_threads(threads)
-    o = O.Oracle(dem.z, dem.rain, O.default_params(side, side), openmp=True, soilT_mode="nonneg")
+    if threads:
+        O.lib(True).orc_set_threads(threads)
+    o = O.Oracle(dem.z, dem.rain, O.default_params(side, side), openmp=threads > 0, soilT_mode="nonneg")
     t0 = time.perf_counter()
     o.step(-1000)
     dt = time.perf_counter() - t0
     return side * side / dt, dt
+
+
+def cpu_baseline(side, serial_side, threads):
+    """OpenMP build on the bench's DEM size, serial build on a smaller sample."""
+    v, dt = cpu_step(side, threads)
+    vs, dts = cpu_step(serial_side, 0)
+    return {"value": v, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"one FP64 timestep of a {side}x{side} DEM from the same generator (C restatement, "
+                      f"OpenMP, {threads} threads), {dt:.1f} s",
+            "parallel_parts": "FlowDirs, plateau rounds, runoff and the elementwise erosion/weathering/update "
+                              "loops; labels, spill heights (Kruskal), flow accumulation and sediment routing "
+                              "run serially",
+            "serial": {"value": vs, "unit": "cells/s", "cores": 1,
+                       "sample": f"one FP64 timestep of a {serial_side}x{serial_side} DEM, serial build, {dts:.1f} s"},
+            "host": platform.processor() or platform.machine()}
 
 
 def main():
@@ -207,7 +225,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8, help="4096^2 DEMs in the batched form")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-DEM forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-side", type=int, default=2048)
+    ap.add_argument("--cpu-side", type=int, default=4096)
+    ap.add_argument("--cpu-serial-side", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB three times (a known-size kernel for counter calibration)")
     ap.add_argument("--tuning", default=None,
@@ -260,12 +279,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        print(f"[bench] CPU baseline (oracle, OpenMP {threads} threads, {args.cpu_side}^2, 1 step)",
-              file=sys.stderr, flush=True)
-        v, dt = cpu_baseline(args.cpu_side, threads)
-        cpu = {"value": v, "unit": "cells/s", "cores": threads, "kind": "port",
-               "sample": f"one FP64 timestep of a {args.cpu_side}x{args.cpu_side} DEM from the same generator "
-                         f"(C restatement, OpenMP), {dt:.1f} s", "host": platform.processor() or platform.machine()}
+        print(f"[bench] CPU baseline (oracle: OpenMP {threads} threads on {args.cpu_side}^2, serial on "
+              f"{args.cpu_serial_side}^2, 1 step each)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_side, args.cpu_serial_side, threads)
 
     roof = roofline(profb, args.steps, atlas_dims[0] * atlas_dims[1], 4, summb[0]["tuning"])
     roof["traffic"] = measured_traffic(roof["kernel"], summb[0]["tuning"], args.batch)
@@ -291,7 +307,14 @@ def main():
         "device": lemhip.device_info(0),
         "tuning": summb[0]["tuning"],
         "last_step": [{k: s[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2", "fa_uncomputed",
-                                         "sed_uncomputed")} for s in summb[:1]],
+                                         "sed_uncomputed", "totwater")} for s in summb[:1]],
+        # input diagnostics per DEM: valid cells the reference's label kernels
+        # also reach from threads past a row end (0: every schedule gives these
+        # labels), and valid cells whose formula soil thickness is negative
+        # (clamped to 0 in this workload)
+        "inputs": {"phantom_cells": [s["phantom_cells"] for s in summb],
+                   "soilT_negative_fraction": [round(s["soilT_negative"] / int((d.z != -9999.0).sum()), 4)
+                                               for s, d in zip(summb, ds)]},
     }
     print(json.dumps(out))
 

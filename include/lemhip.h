@@ -183,12 +183,19 @@ int lemhip_restore(lemhip_ctx *ctx, int slot, const void *buf, size_t bytes);
  * time every candidate on the context's own data (the timed passes are
  * idempotent), keep the fastest, and record the times in tuned_ms:
  * [0..5] flow accumulation per tile shape, [8..10] plateau routing per rounds
- * per launch, [11..13] flow accumulation with batches 4, 8, 32. */
+ * per launch, [11..13] flow accumulation with batches 4, 8, 32.  In the FP32
+ * form under single flow directions, flow accumulation itself does not use
+ * the engine (fp32_integer_fa below); the tile shape then serves sediment
+ * routing. */
 typedef struct {
     int dag_tile_w, dag_tile_h;
     int dag_batch;
     int plateau_rounds_per_launch;
     double tuned_ms[16];
+    /* FP32 form with single flow directions: 1 (default) accumulates flow as
+     * exact integer subtree sums, 0 runs the dependency engine as the other
+     * forms do; both stay within the form's stated bounds */
+    int fp32_integer_fa;
 } lem_tuning;
 int lemhip_get_tuning(lemhip_ctx *ctx, lem_tuning *out);
 int lemhip_set_tuning(lemhip_ctx *ctx, const lem_tuning *t);

@@ -193,9 +193,16 @@ int orc_route_plateaus(int W, int H, const double *zs, const int *mask,
         if (mask[i] == 1 && sp[i] != 0) cand[k++] = i;
     int rounds = 0;
     int changed;
+    /* only candidates ever change, so each round copies only them into the
+     * previous-round buffer; within a round every candidate reads only that
+     * buffer and writes only its own cell (Jacobi), so the candidates can be
+     * processed in parallel */
+    memcpy(prev, sp, sizeof(float) * N);
     do {
         changed = 0;
-        memcpy(prev, sp, sizeof(float) * N);
+#pragma omp parallel for schedule(static)
+        for (long c = 0; c < ncand; c++) prev[cand[c]] = sp[cand[c]];
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : changed)
         for (long c = 0; c < ncand; c++) {
             long self = cand[c];
             int irow = (int)(self / W), icol = (int)(self % W);

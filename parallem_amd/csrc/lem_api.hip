@@ -185,7 +185,7 @@ static void free_all(lemhip_ctx *c)
                     c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
                     c->dmask, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
                     c->dag_ctl, c->partials, c->scal, c->demid, c->ddesc, c->dsc, c->slot_flags, c->labdem,
-                    c->dstage, c->tcache};
+                    c->dstage, c->tcache, c->fx_shift, c->fx_tag_cell, c->fx_tag_sum};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int q = 0; q < 2; q++) {
@@ -287,6 +287,12 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));
     LEM_TRY(A(&c->dag_ctl, 256));
     LEM_TRY(A((void **)&c->partials, 3L * SLOT_KB * (LEM_MAX_SLOTS + 1) * 8));
+    LEM_TRY(A((void **)&c->fx_shift, sizeof(int) * (LEM_MAX_SLOTS + 1)));
+    // tagged cells of the integer flow accumulation: one path per DEM (longer
+    // paths fall back to the dependency engine)
+    c->fx_tag_cap = precision == LEMHIP_FP64 ? 1 : (N / 16 + 4096);
+    LEM_TRY(A((void **)&c->fx_tag_cell, c->fx_tag_cap * sizeof(int)));
+    LEM_TRY(A((void **)&c->fx_tag_sum, c->fx_tag_cap * sizeof(unsigned long long)));
     LEM_TRY(A((void **)&c->scal, sizeof(LemScalars)));
     LEM_CHECK(c, hipMemset(c->scal, 0, sizeof(LemScalars)));
     LEM_TRY(A((void **)&c->ddesc, sizeof(DemDesc) * (LEM_MAX_SLOTS + 1)));
@@ -863,6 +869,7 @@ extern "C" int lemhip_get_tuning(lemhip_ctx *c, lem_tuning *o)
     o->dag_batch = c->dag_batch;
     o->plateau_rounds_per_launch = c->plat_k;
     for (int k = 0; k < 16; k++) o->tuned_ms[k] = c->tune_ms[k];
+    o->fp32_integer_fa = c->fx_enabled ? 1 : 0;
     return LEMHIP_OK;
 }
 
@@ -879,9 +886,11 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     if (t->dag_batch < 1 || t->dag_batch > 256) { c->err = "dag_batch out of range"; return LEMHIP_E_ARG; }
     int k = t->plateau_rounds_per_launch;
     if (k != 4 && k != 8 && k != 16) { c->err = "plateau rounds per launch must be 4, 8 or 16"; return LEMHIP_E_ARG; }
+    if (t->fp32_integer_fa != 0 && t->fp32_integer_fa != 1) { c->err = "fp32_integer_fa must be 0 or 1"; return LEMHIP_E_ARG; }
     c->dag_shape = shape;
     c->dag_batch = t->dag_batch;
     c->plat_k = k;
+    c->fx_enabled = t->fp32_integer_fa != 0;
     return LEMHIP_OK;
 }
 

@@ -139,6 +139,13 @@ struct lemhip_ctx {
     int dag_batch = 16;         // round launches between host checks
     long dag_tiles_max = 0;     // tile arrays are sized for the shape with the most tiles
     double *partials = nullptr; // deterministic reductions
+    // FP32 single-direction flow accumulation in integers (lem_fa32.hip)
+    int *fx_shift = nullptr;          // device: fixed-point shift per slot
+    int hfx_shift[LEM_MAX_SLOTS + 1] = {};
+    int *fx_tag_cell = nullptr;       // cells below the outlet's west neighbour
+    unsigned long long *fx_tag_sum = nullptr;
+    long fx_tag_cap = 0;
+    bool fx_enabled = true;           // false: the dependency engine in every form
     LemScalars *scal = nullptr;   // device
     LemScalars *hscal = nullptr;  // pinned host mirror
     void *stage = nullptr;        // pinned host staging for upload/download
@@ -204,6 +211,7 @@ static inline void prof_end(lemhip_ctx *c, int id, long launches = 1)
 int lem_fd_phase(lemhip_ctx *c, int iter);
 int lem_runoff_phase(lemhip_ctx *c);
 int lem_calcwater(lemhip_ctx *c);
+int lem_fa32_sfd(lemhip_ctx *c, int *fallback);
 int lem_fa_phase(lemhip_ctx *c, int iter);
 int lem_mod_phase(lemhip_ctx *c, int iter);
 int lem_sync_scalars(lemhip_ctx *c);

@@ -63,6 +63,7 @@ struct DemScal {
     int sinkcount;
     int zero_seen;
     int first_iter;       // this step is the slot's first iteration (mfd_simple.cu:336)
+    unsigned long long fx_outlet_sum;   // the outlet's integer flow sum (FP32 single-direction accumulation)
 };
 
 // Per-slot reductions split a slot's rows over slot_kb(h) blocks (row k goes
@@ -93,5 +94,6 @@ struct LemScalars {
     int ncand;            // plateau candidates (mask cells without a downhill neighbour)
     int flag;             // generic change flag for host-driven loops
     int hash_overflow;
+    int w_bad;            // a mask cell's runoff weight is negative or not finite
     unsigned long long lowest_enc;
 };

@@ -1,0 +1,406 @@
+// lem_fa32.hip — flow accumulation of the FP32 form under single flow
+// directions (CalcMFD with one receiver per cell, mfd_simple.cu:33-171) as
+// exact integer subtree sums.
+//
+// With one receiver per cell the donor graph is a forest, and the
+// reference's value of a completed cell, w + max(0, sum of donor fa), is the
+// sum of w over the cell's subtree (every w >= 0, so the clamp never
+// acts).  Each cell's weight is rounded once to a 64-bit fixed-point integer
+// q = round(w * 2^F), with F chosen per DEM from its total runoff so that no
+// sum can exceed 2^61; integer addition is associative, so the subtree sums
+// are exact and do not depend on the order, the tiling or the other DEMs of
+// an atlas.  The only rounding is that of q (relative 2^-F-ish, far below the
+// FP32 form's 1e-6 bound) and of the final conversion to the float pair.
+//
+// Two passes over tiles of TW x TH cells with a small global solve between:
+//   A. each tile sums its own subtrees in LDS by pointer doubling (round t
+//      adds every cell's partial sum to its 2^t-th in-tile ancestor), finds
+//      each cell's in-tile root, and lists its exits (cells whose receiver is
+//      in another tile) with their in-tile sums;
+//   B. the exits form a forest of their own (an exit's parent is the exit
+//      its receiver's in-tile path leads to); the same doubling over that
+//      forest gives every exit its final value;
+//   C. each tile sums its subtrees again with the final values of the exits
+//      that flow into it added at their receivers, and writes fa.
+//
+// Cycles: fd[outlet] = WEST can close a cycle through the outlet (the only
+// possible one: every other link goes strictly downhill or down a plateau's
+// distance field).  Cells on a cycle never complete in the reference and keep
+// fa = w.  The outlet's link is cut and its west neighbour carries a tag of
+// 2^62: tagged cells are exactly the path from the west neighbour to its
+// root.  If the outlet is tagged, that path is the cycle (fa = w); otherwise
+// the path lies downstream of the outlet and gets the outlet's sum added.
+#include "lem_ctx.h"
+
+#define FX_TAG (1ULL << 62)
+#define FX_NONE 0xFFFFu
+
+// direction index of a single ESRI code, -1 for 0 or a multi-bit code
+__device__ __forceinline__ int fx_dir(uint8_t code)
+{
+    switch (code) {
+    case 64: return 0; case 128: return 1; case 1: return 2; case 2: return 3;
+    case 4: return 4; case 8: return 5; case 16: return 6; case 32: return 7;
+    default: return -1;
+    }
+}
+
+// receiver of cell (r, col): a mask cell in direction fd, or -1 (no code,
+// off the grid, nodata, or the outlet whose link is cut)
+__device__ __forceinline__ long fx_recv(int W, int H, int r, int col, long i, const uint8_t *mask, const uint8_t *fd,
+                                        int outlet)
+{
+    const int d = fx_dir(fd[i]);
+    if (d < 0 || i == outlet) return -1;
+    const int nr = r + cdy(d), nc = col + cdx(d);
+    if (nr < 0 || nr >= H || nc < 0 || nc >= W) return -1;
+    const long n = (long)nr * W + nc;
+    return mask[n] ? n : -1;
+}
+
+__device__ __forceinline__ unsigned long long fx_quant(float w, int shift)
+{
+    return (unsigned long long)llrint(ldexp((double)w, shift));
+}
+
+__device__ __forceinline__ float2 fx_value(unsigned long long s, int shift)
+{
+    const double v = ldexp((double)s, -shift);
+    const float hi = (float)v;
+    return make_float2(hi, (float)(v - (double)hi));
+}
+
+// the tagged cell of a slot: the west neighbour of an outlet whose link to
+// it exists (outlet valid, not in column 0, west neighbour valid)
+__device__ __forceinline__ bool fx_tagged(long i, int W, const uint8_t *mask, int outlet)
+{
+    return outlet >= 1 && i == (long)outlet - 1 && (outlet % W) != 0 && mask[outlet] && mask[outlet - 1];
+}
+
+struct FxArgs {
+    int W, H;
+    const uint8_t *mask, *fd, *demid;
+    const float *w;
+    const DemScal *ds;
+    const int *shift;            // per slot
+    // pass A outputs / pass C inputs
+    int *exit_id;                // per cell (exits only)
+    int *xroot;                  // per cell (tile border cells only): exit its in-tile path leads to, or -1
+    unsigned long long *ex_val;  // per exit: in-tile sum (A), final value after the forest solve
+    int *ex_recv;                // per exit: receiver cell
+    int *ex_count;
+    int ex_cap;
+    // pass C outputs
+    float2 *fa;
+    int *tag_cell;
+    unsigned long long *tag_sum;
+    int *tag_count;
+    int tag_cap;
+    DemScal *dsw;
+};
+
+template <int TW, int TH, bool FINAL>
+__global__ void __launch_bounds__(256) k_fx_tile(FxArgs a)
+{
+    constexpr int NC = TW * TH, PER = NC / 256;
+    static_assert(NC % 256 == 0, "tile shape");
+    __shared__ unsigned long long x[NC];
+    __shared__ uint16_t J[NC];
+    __shared__ uint16_t R[FINAL ? 1 : NC];
+    const int W = a.W, H = a.H;
+    const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH;
+    for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+        const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
+        long ext[PER];   // receiver outside the tile (an exit), or -1
+        int slot[PER];
+        bool inb[PER];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int li = threadIdx.x + q * 256;
+            const int r = ty0 + li / TW, col = tx0 + li % TW;
+            ext[q] = -1;
+            slot[q] = 0;
+            inb[q] = r < H && col < W;
+            unsigned long long v = 0;
+            uint16_t j = FX_NONE;
+            if (inb[q]) {
+                const long i = (long)r * W + col;
+                if (a.mask[i]) {
+                    const int s = a.demid ? (int)a.demid[i] : 0;
+                    slot[q] = s;
+                    const int outlet = a.ds[s].outlet;
+                    const int sh = a.shift[s];
+                    v = fx_quant(a.w[i], sh);
+                    if (fx_tagged(i, W, a.mask, outlet)) v += FX_TAG;
+                    const long n = fx_recv(W, H, r, col, i, a.mask, a.fd, outlet);
+                    if (n >= 0) {
+                        const int nr = (int)(n / W) - ty0, nc = (int)(n % W) - tx0;
+                        if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) j = (uint16_t)(nr * TW + nc);
+                        else ext[q] = n;
+                    }
+                    if (FINAL) {
+                        // exits of other tiles that drain into this cell (only
+                        // tile border cells can have them)
+                        const int lr = li / TW, lc = li % TW;
+                        if (lr == 0 || lr == TH - 1 || lc == 0 || lc == TW - 1) {
+#pragma unroll
+                            for (int d = 0; d < 8; d++) {
+                                const int yr = r + cdy(d), yc = col + cdx(d);
+                                const int tr = lr + cdy(d), tc = lc + cdx(d);
+                                if (tr >= 0 && tr < TH && tc >= 0 && tc < TW) continue;   // inside the tile
+                                if (yr < 0 || yr >= H || yc < 0 || yc >= W) continue;
+                                const long y = (long)yr * W + yc;
+                                if (!a.mask[y] || a.fd[y] != donor_code(d)) continue;
+                                const int sy = a.demid ? (int)a.demid[y] : 0;
+                                if (y == a.ds[sy].outlet) continue;                       // cut link
+                                v += a.ex_val[a.exit_id[y]];
+                            }
+                        }
+                    }
+                }
+            }
+            x[li] = v;
+            J[li] = j;
+            if (!FINAL) R[li] = j == FX_NONE ? (uint16_t)li : j;
+        }
+        __syncthreads();
+        // pointer doubling: after round t every cell holds the sum over the
+        // cells within 2^(t+1) - 1 links upstream in the tile
+        for (;;) {
+            unsigned long long v[PER];
+            uint16_t jj[PER], rr[PER];
+            int act = 0;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int li = threadIdx.x + q * 256;
+                const uint16_t j = J[li];
+                if (j != FX_NONE) {
+                    act |= 1 << q;
+                    v[q] = x[li];
+                    jj[q] = J[j];
+                    if (!FINAL) rr[q] = R[R[li]];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                if (!((act >> q) & 1)) continue;
+                const int li = threadIdx.x + q * 256;
+                atomicAdd(&x[J[li]], v[q]);
+                J[li] = jj[q];
+                if (!FINAL) R[li] = rr[q];
+            }
+            if (!__syncthreads_or(act)) break;
+        }
+        // outputs
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            if (!inb[q]) continue;
+            const int li = threadIdx.x + q * 256;
+            const int lr = li / TW, lc = li % TW;
+            const long i = (long)(ty0 + lr) * W + (tx0 + lc);
+            const unsigned long long s = x[li];
+            if (!FINAL) {
+                const bool border = lr == 0 || lr == TH - 1 || lc == 0 || lc == TW - 1;
+                if (ext[q] >= 0) {
+                    const int k = atomicAdd(a.ex_count, 1);
+                    if (k < a.ex_cap) {
+                        a.exit_id[i] = k;
+                        a.ex_val[k] = s;
+                        a.ex_recv[k] = (int)ext[q];
+                    }
+                }
+                if (border) {
+                    // the in-tile root is an exit exactly when it has a receiver
+                    // outside the tile: read back from the root's own state
+                    const int root = R[li];
+                    const int rr = ty0 + root / TW, rc = tx0 + root % TW;
+                    const long ri = (long)rr * W + rc;
+                    long rn = -1;
+                    if (a.mask[ri]) {
+                        const int s2 = a.demid ? (int)a.demid[ri] : 0;
+                        rn = fx_recv(W, H, rr, rc, ri, a.mask, a.fd, a.ds[s2].outlet);
+                    }
+                    a.xroot[i] = rn >= 0 ? (int)ri : -1;
+                }
+            } else {
+                if (!a.mask[i]) {
+                    a.fa[i] = make_float2(0.f, 0.f);
+                    continue;
+                }
+                const int s2 = slot[q];
+                if (i == a.ds[s2].outlet) a.dsw[s2].fx_outlet_sum = s;
+                if (s >= FX_TAG) {
+                    const int k = atomicAdd(a.tag_count, 1);
+                    if (k < a.tag_cap) {
+                        a.tag_cell[k] = (int)i;
+                        a.tag_sum[k] = s;
+                    }
+                } else {
+                    a.fa[i] = fx_value(s, a.shift[s2]);
+                }
+            }
+        }
+    }
+}
+
+// parent of each exit in the exit forest
+__global__ void k_fx_parent(int n, const int *ex_recv, const int *xroot, const int *exit_id, int *P)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int xr = xroot[ex_recv[k]];
+        P[k] = xr >= 0 ? exit_id[xr] : -1;
+    }
+}
+
+// one doubling round over the exit forest in two launches: snapshot, then add
+__global__ void k_fx_gather(int n, const unsigned long long *val, const int *P, unsigned long long *tv, int *tp)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int p = P[k];
+        tv[k] = val[k];
+        tp[k] = p >= 0 ? P[p] : -1;
+    }
+}
+
+__global__ void k_fx_scatter(int n, unsigned long long *val, int *P, const unsigned long long *tv, const int *tp,
+                             int *flag)
+{
+    bool more = false;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int p = P[k];
+        if (p >= 0) {
+            atomicAdd(&val[p], tv[k]);
+            P[k] = tp[k];
+            more |= tp[k] >= 0;
+        }
+    }
+    if (__any(more) && (threadIdx.x & 63) == 0) *flag = 1;
+}
+
+// tagged cells: the cycle through the outlet (fa = w, never completed), or
+// the path below the outlet (the outlet's sum added)
+__global__ void k_fx_tags(const int *count, int cap, const int *tag_cell, const unsigned long long *tag_sum,
+                          const uint8_t *demid, const DemScal *ds, const int *shift, const float *w, float2 *fa,
+                          int *uncomputed)
+{
+    const int n = *count < cap ? *count : cap;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int i = tag_cell[k];
+        const int s = demid ? (int)demid[i] : 0;
+        const unsigned long long o = ds[s].fx_outlet_sum;
+        if (o >= FX_TAG) {
+            fa[i] = make_float2(w[i], 0.f);
+            atomicAdd(uncomputed, 1);
+        } else {
+            fa[i] = fx_value(tag_sum[k] - FX_TAG + o, shift[s]);
+        }
+    }
+}
+
+static int fx_grid(long n)
+{
+    long g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (int)g;
+}
+
+// 0: done; 1: inputs the integer form cannot represent exactly (a negative
+// or non-finite runoff weight, or too many tagged cells) — the caller runs
+// the dependency engine instead
+int lem_fa32_sfd(lemhip_ctx *c, int *fallback)
+{
+    *fallback = 0;
+    LEM_TRY(lem_read_slots(c));
+    LEM_CHECK(c, hipMemcpyAsync(&c->hint[30], &c->scal->w_bad, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    if (c->hint[30]) { *fallback = 1; return LEMHIP_OK; }
+    // per-slot shift: sums stay below 2^61 (the rounding adds at most N/2)
+    for (int j = 0; j < c->nslot; j++) {
+        const double tw = c->hdsc[j].totwater;
+        int sh = 61;
+        if (c->hdesc[j].w > 0) {
+            if (!(tw >= 0.0) || tw > 1e300) { *fallback = 1; return LEMHIP_OK; }
+            if (tw > 0.0) sh = 61 - (int)ceil(log2(tw * (1 + 1e-12)));
+            if (sh > 1000) sh = 1000;
+        }
+        c->hfx_shift[j] = sh;
+    }
+    LEM_CHECK(c, hipMemcpyAsync(c->fx_shift, c->hfx_shift, sizeof(int) * c->nslot, hipMemcpyHostToDevice, c->s));
+    const long N = c->N;
+    FxArgs a;
+    a.W = c->W;
+    a.H = c->H;
+    a.mask = c->mask;
+    a.fd = c->fd;
+    a.demid = c->demid;
+    a.w = (const float *)c->w;
+    a.ds = c->dsc;
+    a.shift = c->fx_shift;
+    a.exit_id = reinterpret_cast<int *>(c->scan_tmp);
+    a.xroot = c->ws;
+    unsigned long long *tv = reinterpret_cast<unsigned long long *>(c->z2);
+    a.ex_val = tv + N / 2;
+    a.ex_recv = c->cand;
+    a.ex_count = &c->scal->nedges;
+    a.ex_cap = (int)(N / 2);
+    a.fa = reinterpret_cast<float2 *>(c->fa);
+    a.tag_cell = c->fx_tag_cell;
+    a.tag_sum = c->fx_tag_sum;
+    a.tag_count = &c->scal->flag;
+    a.tag_cap = (int)c->fx_tag_cap;
+    a.dsw = c->dsc;
+    constexpr int TW = 64, TH = 32;
+    const long ntiles = (long)((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
+    int bpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fx_tile<TW, TH, false>, 256, 0) != hipSuccess || bpc < 1)
+        bpc = 2;
+    int G = bpc * c->ncu;
+    if (G > ntiles) G = (int)ntiles;
+    // A: in-tile sums, exits, in-tile roots of border cells
+    LEM_CHECK(c, hipMemsetAsync(a.ex_count, 0, sizeof(int), c->s));
+    k_fx_tile<TW, TH, false><<<G, 256, 0, c->s>>>(a);
+    LEM_LAUNCHED(c);
+    LEM_CHECK(c, hipMemcpyAsync(&c->hint[31], a.ex_count, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    const int nex = c->hint[31];
+    // the exit values and their snapshot share the 8-byte scratch of N cells;
+    // exits are tile border cells (9% of a full tile), so only grids of a few
+    // cells could have more
+    if ((long)nex > N / 2) { *fallback = 1; return LEMHIP_OK; }
+    // B: the exit forest
+    if (nex > 0) {
+        int *P = reinterpret_cast<int *>(c->sp0), *tp = reinterpret_cast<int *>(c->sp1);
+        k_fx_parent<<<fx_grid(nex), 256, 0, c->s>>>(nex, a.ex_recv, a.xroot, a.exit_id, P);
+        LEM_LAUNCHED(c);
+        for (int round = 0;; round++) {
+            LEM_CHECK(c, hipMemsetAsync(&c->scal->nlabels, 0, sizeof(int), c->s));
+            for (int q = 0; q < 2; q++) {
+                k_fx_gather<<<fx_grid(nex), 256, 0, c->s>>>(nex, a.ex_val, P, tv, tp);
+                k_fx_scatter<<<fx_grid(nex), 256, 0, c->s>>>(nex, a.ex_val, P, tv, tp, &c->scal->nlabels);
+            }
+            LEM_LAUNCHED(c);
+            LEM_CHECK(c, hipMemcpyAsync(&c->hint[32], &c->scal->nlabels, sizeof(int), hipMemcpyDeviceToHost, c->s));
+            LEM_CHECK(c, hipStreamSynchronize(c->s));
+            if (!c->hint[32]) break;
+            if (round > 32) { c->err = "exit forest did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        }
+    }
+    // C: final sums, fa, tagged cells
+    LEM_CHECK(c, hipMemsetAsync(a.tag_count, 0, sizeof(int), c->s));
+    k_fx_tile<TW, TH, true><<<G, 256, 0, c->s>>>(a);
+    LEM_LAUNCHED(c);
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->fa_uncomputed, 0, sizeof(int), c->s));
+    k_fx_tags<<<64, 256, 0, c->s>>>(a.tag_count, a.tag_cap, a.tag_cell, a.tag_sum, c->demid, c->dsc, c->fx_shift,
+                                    a.w, a.fa, &c->scal->fa_uncomputed);
+    LEM_LAUNCHED(c);
+    LEM_CHECK(c, hipMemcpyAsync(&c->hint[33], a.tag_count, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipMemcpyAsync(&c->hint[34], &c->scal->fa_uncomputed, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    if (c->hint[33] > a.tag_cap) { *fallback = 1; return LEMHIP_OK; }
+    c->h_fa_unc = c->hint[34];
+    c->dag_rounds_last = 0;
+    return LEMHIP_OK;
+}

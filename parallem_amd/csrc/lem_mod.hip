@@ -48,11 +48,14 @@ static int grid1d(long n, int block = 256, int cap = 8192)
 
 // calcrunoffweight (runoffweight.cu:21-58); also seeds fa = w on mask cells,
 // the value a cell keeps if it never completes (mfd_simple.cu:72)
+// bad: raised when a mask cell's weight is negative or not finite (the FP32
+// form's integer accumulation needs weights in [0, inf))
 template <typename T>
 __global__ void k_runoff(long N, const uint8_t *__restrict__ mask, const T *__restrict__ rain,
                          const T *__restrict__ stone, const T *__restrict__ TotB, T *__restrict__ soilM,
-                         T *__restrict__ w, typename FaS<T>::type *__restrict__ fa)
+                         T *__restrict__ w, typename FaS<T>::type *__restrict__ fa, int *bad)
 {
+    bool b = false;
     for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
         if (mask[self] == 0) {
             w[self] = (T)-9999;
@@ -69,7 +72,9 @@ __global__ void k_runoff(long N, const uint8_t *__restrict__ mask, const T *__re
         if (sm > (T)0.5) sm = (T)0.5;
         soilM[self] = sm;
         fa_set(&fa[self], wv);
+        b |= !(wv >= (T)0) || isinf(wv);
     }
+    if (__any(b) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
 }
 
 // ------------------------------------------------ MOD part 1 (pre-sediment)
@@ -574,8 +579,10 @@ static int runoff_t(lemhip_ctx *c)
 {
     using FS = typename FaS<T>::type;
     prof_begin(c, PROF_RUNOFF);
+    LEM_CHECK(c, hipMemsetAsync(&c->scal->w_bad, 0, sizeof(int), c->s));
     k_runoff<T><<<grid1d(c->N), 256, 0, c->s>>>(c->N, c->mask, (const T *)c->rain, (const T *)c->stone,
-                                                 (const T *)c->TotB, (T *)c->soilM, (T *)c->w, (FS *)c->fa);
+                                                 (const T *)c->TotB, (T *)c->soilM, (T *)c->w, (FS *)c->fa,
+                                                 &c->scal->w_bad);
     prof_end(c, PROF_RUNOFF);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
@@ -609,7 +616,12 @@ static int fa_t(lemhip_ctx *c)
     using FS = typename FaS<T>::type;
     if (c->tune_dag_pending) LEM_TRY(lem_tune_dag(c));
     prof_begin(c, PROF_FA);
-    LEM_TRY(lem_fa_dag<T>(c));
+    // FP32 form, single flow directions: exact integer subtree sums
+    // (lem_fa32.hip); otherwise, and for inputs those cannot represent, the
+    // dependency engine
+    int fallback = 1;
+    if (sizeof(T) == 4 && c->p.flowdirectiontype == 0 && c->fx_enabled) LEM_TRY(lem_fa32_sfd(c, &fallback));
+    if (fallback) LEM_TRY(lem_fa_dag<T>(c));
     prof_end(c, PROF_FA, c->dag_rounds_last);
     // FA_max and the first-iteration outlet rule (mfd_simple.cu:332-357)
     int ns = c->nslot;

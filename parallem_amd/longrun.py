@@ -23,8 +23,7 @@ from . import lemhip, synth
 
 
 def device_used_gb() -> float:
-    import torch
-    free, total = torch.cuda.mem_get_info(0)
+    free, total = lemhip.mem_info(0)
     return (total - free) / 1e9
 
 

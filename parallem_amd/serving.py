@@ -14,6 +14,14 @@ Per step each live request gets SetClimateMatrices(k) semantics with its own
 forcing (Data.cpp:231-271) before the step and copytolastclimate
 (memory_dev.cu:363-373) after it, the canonical rule for forcing series.
 
+Admission walks the arrived requests in arrival order and admits every one
+that fits, so a large DEM waiting for room does not hold back small ones
+behind it; once the oldest waiting request has been passed over for
+`max_bypass` iterations, nothing behind it is admitted until it fits (no
+starvation).  The footprint per cell comes from the library
+(lemhip_bytes_per_cell): the fixed arrays of an atlas context plus the bound
+on the depression-routing arrays.
+
 Latency is the time between a request's consecutive step completions (the
 first step counts from admission); throughput is cells stepped per second.
 """
@@ -27,8 +35,13 @@ import numpy as np
 
 from . import lemhip, synth
 
-# device bytes per atlas cell of the two precision forms (state + scratch)
-BYTES_PER_CELL = {lemhip.FP64: 309, lemhip.FP32_KAHAN: 193}
+
+
+def bytes_per_cell(precision: int) -> float:
+    """Device bytes per atlas cell of a precision form (state, scratch and
+    the bound on the depression-routing arrays), from the library."""
+    fixed, grow = lemhip.bytes_per_cell(precision, atlas=True)
+    return fixed + grow
 
 
 @dataclasses.dataclass
@@ -72,9 +85,11 @@ def make_requests(count: int, rate: float, seed: int, lo: int = 256, hi: int = 4
 class Server:
     def __init__(self, width: int, height: int, precision: int = lemhip.FP32_KAHAN, device: int = 0,
                  hbm_budget: float | None = None, flowtype: int = 0, soilT_mode: str = "nonneg",
-                 keep_fields: tuple = ("dem", "fa", "soilT")):
+                 keep_fields: tuple = ("dem", "fa", "soilT"), max_bypass: int = 8):
         self.precision = precision
-        self.budget = hbm_budget if hbm_budget is not None else BYTES_PER_CELL[precision] * width * height
+        self.bpc = bytes_per_cell(precision)
+        self.budget = hbm_budget if hbm_budget is not None else self.bpc * width * height
+        self.max_bypass = max_bypass
         self.params = lemhip.default_params(width, height, flowtype)
         self.atlas = lemhip.LemAtlas(self.params, width, height, precision, device)
         self.soilT_mode = soilT_mode
@@ -87,13 +102,13 @@ class Server:
     @staticmethod
     def sized_for(max_side: int, precision: int, budget_bytes: float, **kw) -> "Server":
         """An atlas as large as the budget allows, at least one max-size DEM."""
-        cells = budget_bytes / BYTES_PER_CELL[precision]
+        cells = budget_bytes / bytes_per_cell(precision)
         side = max(max_side + 1, int(math.sqrt(cells)))
         return Server(side, side, precision, hbm_budget=budget_bytes, **kw)
 
     def footprint(self, r: Request) -> float:
         h, w = r.dem.z.shape
-        return BYTES_PER_CELL[self.precision] * (w + 1) * (h + 1)
+        return self.bpc * (w + 1) * (h + 1)
 
     def try_admit(self, r: Request, now: float) -> bool:
         if self.used + self.footprint(r) > self.budget:
@@ -137,13 +152,19 @@ class Server:
         t0 = time.perf_counter()
         now = (lambda: time.perf_counter() - t0)
         finished = []
+        bypassed = 0   # iterations the oldest waiting request has been passed over
         while pending or self.live:
             t = now()
-            # admit in arrival order; a request that does not fit blocks the ones behind it (no starvation)
-            while pending and (not realtime or pending[0].arrival <= t):
-                if not self.try_admit(pending[0], t):
-                    break
-                pending.pop(0)
+            arrived = [r for r in pending if not realtime or r.arrival <= t]
+            for k, r in enumerate(arrived):
+                if self.try_admit(r, t):
+                    pending.remove(r)
+                    if k == 0:
+                        bypassed = 0
+                elif k == 0 and bypassed >= self.max_bypass:
+                    break      # the oldest has waited long enough: keep the room for it
+            if pending and arrived and pending[0] is arrived[0]:
+                bypassed += 1
             if not self.live:
                 if pending and realtime and pending[0].arrival > t:
                     time.sleep(min(0.01, pending[0].arrival - t))

@@ -131,6 +131,7 @@ def _assert_fp32_near_oracle(dem, flowtype, steps, where):
         ok = (a > 0) & (a < 3e38)
         worst_fa = max(worst_fa, float(np.max(np.abs(b[ok] - a[ok]) / a[ok])))
         worst_z = max(worst_z, float(np.max(np.abs(c.download("dem")[m] - orc["dem"][m]))))
+        assert c.summary()["fa_uncomputed"] == orc.scalars()["fa_uncomputed"], f"{where} step {k}: cycle cells"
     print(f"\n[fp32 vs oracle] {where}: fa max rel {worst_fa:.3e}, max |dz| {worst_z:.3e} m over {steps} steps")
     assert worst_fa <= 1e-6
     assert worst_z <= 1e-4
@@ -147,3 +148,50 @@ def test_fp32_against_the_oracle_fbm_512_mfd():
 
 def test_fp32_against_the_oracle_1024_one_step():
     _assert_fp32_near_oracle(synth.fbm_4096(side=1024), 0, 1, "1024^2 SFD")
+
+
+def test_integer_accumulation_equals_the_engine():
+    """FP32 form, single flow directions: the exact integer subtree sums and
+    the dependency engine's compensated float sums agree to 1e-9 relative on
+    every cell, the outlet cycle included (runiniter = 0 moves the outlet onto
+    the largest river, whose west neighbour then drains back into it)."""
+    dem = synth.fbm_512(side=640, seed_offset=61)
+    H, W = dem.z.shape
+    ctx = {}
+    for integer in (1, 0):
+        c = lemhip.LemContext(lemhip.default_params(W, H, 0, runiniter=0), lemhip.FP32_KAHAN, 0)
+        c.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+        c.set_tuning(fp32_integer_fa=integer)
+        ctx[integer] = c
+    for k in range(3):
+        for c in ctx.values():
+            c.step(-1000 + k)
+        a, b = ctx[1].download("fa"), ctx[0].download("fa")
+        m = ctx[1].download("mask") == 1
+        assert np.array_equal(ctx[1].download("fd"), ctx[0].download("fd"))
+        rel = np.abs(a[m] - b[m]) / np.maximum(b[m], 1e-30)
+        assert float(rel.max()) <= 1e-9, f"step {k}: {float(rel.max())}"
+        assert ctx[1].summary()["fa_uncomputed"] == ctx[0].summary()["fa_uncomputed"]
+        assert np.array_equal(ctx[1].download("dem"), ctx[0].download("dem")) or k > 0
+    print(f"\n[fp32] integer vs engine accumulation: cycle cells {ctx[1].summary()['fa_uncomputed']}")
+    assert ctx[1].tuning()["fp32_integer_fa"] == 1 and ctx[0].tuning()["fp32_integer_fa"] == 0
+
+
+def test_integer_accumulation_falls_back_on_negative_weights():
+    """A negative rain cell gives a negative runoff weight, which the
+    integer sums cannot hold: the step falls back to the engine and still
+    matches the FP64 form."""
+    dem = synth.fbm_512(side=200, seed_offset=62)
+    rain = dem.rain.copy()
+    valid = dem.z != -9999.0
+    rain[np.flatnonzero(valid)[500]] = -50.0
+    H, W = dem.z.shape
+    c64, c32 = (lemhip.LemContext(lemhip.default_params(W, H, 0), p, 0) for p in (lemhip.FP64, lemhip.FP32_KAHAN))
+    for c in (c64, c32):
+        c.init_state(dem.z, rain, soilT_mode="nonneg")
+        c.flow_direction(-1000)
+        c.runoff()
+        c.flow_accum(-1000)
+    a, b = c64.download("fa"), c32.download("fa")
+    m = valid.reshape(-1) & (np.abs(a) > 1e-12)
+    assert float(np.max(np.abs(b[m] - a[m]) / np.abs(a[m]))) <= 1e-6
