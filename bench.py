@@ -123,23 +123,61 @@ def run_single(dem, precision, steps, warmup, tuning=None):
     return total, lat, prof, summ
 
 
-def run_batched(ds, precision, steps, warmup, tuning=None):
+class Group:
+    """Several atlases stepped together (lemhip_step_group), driven through
+    the same timing helper as one context."""
+
+    def __init__(self, atlases):
+        self.atlases = atlases
+
+    def autotune(self):
+        for a in self.atlases:
+            a.autotune()
+
+    def profile(self, on):
+        for a in self.atlases:
+            a.profile(on)
+
+    def profile_read(self):
+        """Phase times summed over the atlases (each on its own stream)."""
+        out = {}
+        for a in self.atlases:
+            for k, (ms, n) in a.profile_read().items():
+                m0, n0 = out.get(k, (0.0, 0))
+                out[k] = (m0 + ms, n0 + n)
+        return out
+
+    def step(self):
+        from parallem_amd import lemhip
+        lemhip.step_group(self.atlases)
+
+
+def run_batched(ds, precision, steps, warmup, tuning=None, groups=1):
+    """The DEMs split over `groups` atlases (each a grid of DEM rectangles with
+    the fewest empty slots, then the squarest), stepped together."""
     from parallem_amd import lemhip
-    # the grid of DEM rectangles with the fewest empty slots, then the squarest
-    n = len(ds)
-    cols = min(range(1, n + 1), key=lambda c: (c * -(-n // c), abs(c - -(-n // c))))
-    rows = -(-n // cols)
-    aw, ah = cols * (SIDE + 1), rows * (SIDE + 1)
-    atlas = lemhip.LemAtlas(lemhip.default_params(aw, ah, flowtype=0), aw, ah, precision, 0)
-    slots = [atlas.add(d.z, d.rain, soilT_mode="nonneg") for d in ds]
-    assert all(s is not None for s in slots)
-    if tuning:
-        atlas.set_tuning(**tuning)
-    total, lat, prof = timed(atlas.step, steps, warmup, atlas, autotune=tuning is None)
-    summ = [atlas.summary(s) for s in slots]
-    summ[0]["tuning"] = atlas.tuning()
-    atlas.close()
-    return total, lat, prof, summ, (aw, ah)
+    parts = [ds[g::groups] for g in range(groups)]
+    atlases, slots, dims = [], [], []
+    for part in parts:
+        n = len(part)
+        cols = min(range(1, n + 1), key=lambda c: (c * -(-n // c), abs(c - -(-n // c))))
+        rows = -(-n // cols)
+        aw, ah = cols * (SIDE + 1), rows * (SIDE + 1)
+        atlas = lemhip.LemAtlas(lemhip.default_params(aw, ah, flowtype=0), aw, ah, precision, 0)
+        sl = [atlas.add(d.z, d.rain, soilT_mode="nonneg") for d in part]
+        assert all(s is not None for s in sl)
+        if tuning:
+            atlas.set_tuning(**tuning)
+        atlases.append(atlas)
+        slots.append(sl)
+        dims.append((aw, ah))
+    grp = Group(atlases)
+    total, lat, prof = timed(grp.step, steps, warmup, grp, autotune=tuning is None)
+    summ = [a.summary(s) for a, sl in zip(atlases, slots) for s in sl]
+    summ[0]["tuning"] = atlases[0].tuning()
+    for a in atlases:
+        a.close()
+    return total, lat, prof, summ, dims
 
 
 def phase_ms(prof, steps):
@@ -223,6 +261,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=8, help="4096^2 DEMs in the batched form")
+    ap.add_argument("--groups", type=int, default=2,
+                    help="atlases the batch is split over, stepped together on their own streams")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-DEM forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-side", type=int, default=4096)
@@ -256,7 +296,7 @@ def main():
 
     print("[bench] batched FP32+Kahan form", file=sys.stderr, flush=True)
     tb, latb, profb, summb, atlas_dims = run_batched(ds[:args.batch], lemhip.FP32_KAHAN, args.steps, args.warmup,
-                                                     tuning=fixed)
+                                                     tuning=fixed, groups=args.groups)
     value = cells_b * args.steps / tb
 
     single32 = plain = None
@@ -283,7 +323,7 @@ def main():
               f"{args.cpu_serial_side}^2, 1 step each)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_side, args.cpu_serial_side, threads)
 
-    roof = roofline(profb, args.steps, atlas_dims[0] * atlas_dims[1], 4, summb[0]["tuning"])
+    roof = roofline(profb, args.steps, sum(w * h for w, h in atlas_dims), 4, summb[0]["tuning"])
     roof["traffic"] = measured_traffic(roof["kernel"], summb[0]["tuning"], args.batch)
     if roof["traffic"] is not None:
         roof["traffic_note"] = "HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"
@@ -293,8 +333,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 dem/soilT, f32x2 fa)",
         "data": "synthetic",
         "config": {"workload": f"batch of {args.batch} independent {SIDE}x{SIDE} fBm DEMs (400 pits, 100 plateaus, "
-                               "lognormal rain each) in one atlas grid, SFD, FP32+Kahan timestep",
-                   "dems": args.batch, "side": SIDE, "cells": cells_b, "atlas": list(atlas_dims),
+                               f"lognormal rain each) in {args.groups} atlas grid(s) stepped together, SFD, "
+                               "FP32+Kahan timestep",
+                   "dems": args.batch, "side": SIDE, "cells": cells_b, "atlases": [list(d) for d in atlas_dims],
                    "flowdirectiontype": 0, "precision": "fp32_kahan"},
         "p50_ms": float(np.percentile(latb, 50) * 1e3), "p99_ms": float(np.percentile(latb, 99) * 1e3),
         "step_roofline_frac": value * B_ALG["fp32"] / (HBM_PEAK_GBS * 1e9),

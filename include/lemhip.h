@@ -174,6 +174,15 @@ int lemhip_snapshot_size(lemhip_ctx *ctx, int slot, size_t *bytes);
 int lemhip_snapshot(lemhip_ctx *ctx, int slot, void *buf, size_t bytes);
 int lemhip_restore(lemhip_ctx *ctx, int slot, const void *buf, size_t bytes);
 
+/* Steps several contexts at once (no reference counterpart): each from its
+ * own host thread on its own stream, so their kernels overlap on the device.
+ * iters[k] is the iteration of single-DEM context k; slot_iters[k] the
+ * per-slot iterations of atlas context k (as lemhip_step_slots takes them).
+ * Contexts must be distinct and on the same device or on different ones;
+ * returns the first failure's code (each context's lemhip_last_error says
+ * more). */
+int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const int *const *slot_iters);
+
 /* Launch tuning (no reference counterpart).  The dependency engine of the
  * two accumulation passes runs in tiles of dag_tile_w x dag_tile_h cells
  * (64x16, 32x32, 16x64, 128x8, 64x32 or 32x64) and checks convergence on the

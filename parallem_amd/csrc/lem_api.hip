@@ -7,6 +7,7 @@
 #include <new>
 #include <vector>
 #include <type_traits>
+#include <thread>
 
 static int grid1d(long n, int block = 256, int cap = 8192)
 {
@@ -912,6 +913,38 @@ extern "C" int lemhip_step_slots(lemhip_ctx *c, const int *iters)
 {
     if (!c || !iters) return LEMHIP_E_ARG;
     return step_impl(c, iters, 0);
+}
+
+// Several contexts stepped at once, each from its own host thread on its own
+// stream: their kernel sequences overlap on the device, so one context's
+// latency-bound stretches (the late rounds of the dependency passes, which
+// keep few compute units busy) run beside another's bandwidth-bound kernels.
+extern "C" int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const int *const *slot_iters)
+{
+    if (!ctxs || n <= 0 || n > 64) return LEMHIP_E_ARG;
+    for (int k = 0; k < n; k++) {
+        if (!ctxs[k]) return LEMHIP_E_ARG;
+        if (ctxs[k]->atlas ? !(slot_iters && slot_iters[k]) : !iters) return LEMHIP_E_ARG;
+        for (int q = 0; q < k; q++)
+            if (ctxs[q] == ctxs[k]) return LEMHIP_E_ARG;   // one thread per context
+    }
+    std::vector<int> rc(n, LEMHIP_OK);
+    auto one = [&](int k) {
+        lemhip_ctx *c = ctxs[k];
+        rc[k] = c->atlas ? step_impl(c, slot_iters[k], 0) : step_impl(c, nullptr, iters[k]);
+    };
+    if (n == 1) {
+        one(0);
+    } else {
+        std::vector<std::thread> th;
+        th.reserve(n - 1);
+        for (int k = 1; k < n; k++) th.emplace_back(one, k);
+        one(0);
+        for (auto &t : th) t.join();
+    }
+    for (int k = 0; k < n; k++)
+        if (rc[k] != LEMHIP_OK) return rc[k];
+    return LEMHIP_OK;
 }
 
 extern "C" int lemhip_profile_enable(lemhip_ctx *c, int on)

@@ -43,6 +43,7 @@ EXPORTS = [
     "lemhip_slot_get_outlet", "lemhip_slot_set_outlet", "lemhip_slot_set_climate", "lemhip_slot_summary",
     "lemhip_step_slots", "lemhip_get_tuning", "lemhip_set_tuning", "lemhip_autotune",
     "lemhip_mem_info", "lemhip_bytes_per_cell", "lemhip_snapshot_size", "lemhip_snapshot", "lemhip_restore",
+    "lemhip_step_group",
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
@@ -160,6 +161,7 @@ def lib() -> C.CDLL:
     L.lemhip_snapshot_size.argtypes = [vp, C.c_int, C.POINTER(C.c_size_t)]
     L.lemhip_snapshot.argtypes = [vp, C.c_int, vp, C.c_size_t]
     L.lemhip_restore.argtypes = [vp, C.c_int, vp, C.c_size_t]
+    L.lemhip_step_group.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int))]
     _lib = L
     return L
 
@@ -266,6 +268,7 @@ class LemContext(_Handle):
         self.precision = precision
         self.W, self.H = params.width, params.height
         self.N = self.W * self.H
+        self.it = params.start_iter      # next iteration for step_group
         self._open(lambda ph: self.L.lemhip_create(C.byref(params), precision, device, ph), "lemhip_create")
 
     # --- state -----------------------------------------------------------
@@ -429,6 +432,29 @@ class LemAtlas(_Handle):
         """Resume an open slot from a checkpoint; its next step is `iteration`."""
         self._restore(slot, snap)
         self.iters[slot] = iteration
+
+
+def step_group(handles: list) -> None:
+    """One timestep of several contexts at once (lemhip_step_group): each
+    runs from its own host thread on its own stream, so their kernels overlap.
+    Single-DEM contexts step at their `it` attribute (then incremented),
+    atlases at their per-slot iterations."""
+    L = lib()
+    n = len(handles)
+    hs = (C.c_void_p * n)(*[h.h.value for h in handles])
+    its = (C.c_int * n)(*[getattr(h, "it", 0) for h in handles])
+    IP = C.POINTER(C.c_int)
+    slot_its = (IP * n)(*[h.iters.ctypes.data_as(IP) if isinstance(h, LemAtlas) else IP() for h in handles])
+    rc = L.lemhip_step_group(hs, n, its, slot_its)
+    if rc != 0:
+        msgs = "; ".join(L.lemhip_last_error(h.h).decode() for h in handles)
+        raise LemError(f"lemhip_step_group: {rc} {msgs}")
+    for h in handles:
+        if isinstance(h, LemAtlas):
+            for s in h.shapes:
+                h.iters[s] += 1
+        else:
+            h.it = getattr(h, "it", 0) + 1
 
 
 # reference-named entry points (lem.cpp:84-96)
