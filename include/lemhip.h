@@ -205,6 +205,12 @@ typedef struct {
      * exact integer subtree sums, 0 runs the dependency engine as the other
      * forms do; both stay within the form's stated bounds */
     int fp32_integer_fa;
+    /* 1: the straight-line kernel sequences of the erosion phase (part 1 with
+     * its totals; the totals, part 2 and the outlet bookkeeping) are captured
+     * once into HIP graphs and replayed each step (recaptured when the band
+     * of rows or the slot count changes); 0 (default): launched directly.
+     * Results are identical. */
+    int use_graphs;
 } lem_tuning;
 int lemhip_get_tuning(lemhip_ctx *ctx, lem_tuning *out);
 int lemhip_set_tuning(lemhip_ctx *ctx, const lem_tuning *t);

@@ -199,6 +199,10 @@ static void free_all(lemhip_ctx *c)
     if (c->hflags) (void)hipHostFree(c->hflags);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->hint) (void)hipHostFree(c->hint);
+    for (GraphSeg &g : c->gseg) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     if (c->prof_ready)
         for (int k = 0; k < PROF_COUNT; k++)
             for (int q = 0; q < PROF_MAXP; q++) {
@@ -871,6 +875,7 @@ extern "C" int lemhip_get_tuning(lemhip_ctx *c, lem_tuning *o)
     o->plateau_rounds_per_launch = c->plat_k;
     for (int k = 0; k < 16; k++) o->tuned_ms[k] = c->tune_ms[k];
     o->fp32_integer_fa = c->fx_enabled ? 1 : 0;
+    o->use_graphs = c->use_graphs ? 1 : 0;
     return LEMHIP_OK;
 }
 
@@ -888,6 +893,8 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     int k = t->plateau_rounds_per_launch;
     if (k != 4 && k != 8 && k != 16) { c->err = "plateau rounds per launch must be 4, 8 or 16"; return LEMHIP_E_ARG; }
     if (t->fp32_integer_fa != 0 && t->fp32_integer_fa != 1) { c->err = "fp32_integer_fa must be 0 or 1"; return LEMHIP_E_ARG; }
+    if (t->use_graphs != 0 && t->use_graphs != 1) { c->err = "use_graphs must be 0 or 1"; return LEMHIP_E_ARG; }
+    c->use_graphs = t->use_graphs != 0;
     c->dag_shape = shape;
     c->dag_batch = t->dag_batch;
     c->plat_k = k;

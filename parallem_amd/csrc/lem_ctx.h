@@ -45,6 +45,14 @@ struct ShelfPack {
     void release(int x, int y, int w, int h);
 };
 
+// A straight-line sequence of launches on the context stream, captured once
+// into a HIP graph and replayed while its launch shapes stay the same (key).
+struct GraphSeg {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long key = -1;
+};
+
 struct lemhip_ctx {
     lem_params p;
     int prec = LEMHIP_FP64;
@@ -146,6 +154,11 @@ struct lemhip_ctx {
     unsigned long long *fx_tag_sum = nullptr;
     long fx_tag_cap = 0;
     bool fx_enabled = true;           // false: the dependency engine in every form
+    // MOD's straight-line segments (part 1 with its totals; the totals,
+    // part 2 and the outlet bookkeeping) replayed from HIP graphs
+    bool use_graphs = false;
+    bool capturing = false;
+    GraphSeg gseg[2];
     LemScalars *scal = nullptr;   // device
     LemScalars *hscal = nullptr;  // pinned host mirror
     void *stage = nullptr;        // pinned host staging for upload/download
@@ -192,13 +205,13 @@ struct lemhip_ctx {
 
 static inline void prof_begin(lemhip_ctx *c, int id)
 {
-    if (!c->prof_on) return;
+    if (!c->prof_on || c->capturing) return;
     ProfSlot &p = c->prof[id];
     if (p.n < PROF_MAXP) (void)hipEventRecord(p.a[p.n], c->s);
 }
 static inline void prof_end(lemhip_ctx *c, int id, long launches = 1)
 {
-    if (!c->prof_on) return;
+    if (!c->prof_on || c->capturing) return;
     ProfSlot &p = c->prof[id];
     if (p.n < PROF_MAXP) {
         (void)hipEventRecord(p.b[p.n], c->s);

@@ -22,10 +22,15 @@ template <typename T> struct FaStore;
 template <> struct FaStore<double> { using type = double; };
 template <> struct FaStore<float> { using type = float2; };
 
-// float-float (double-word) arithmetic for the compensated FP32 form
+// float-float (double-word) arithmetic for the compensated FP32 form.  A sum
+// past the float range saturates to (inf, 0): the error terms would form
+// inf - inf = NaN, while the FP64 form keeps a huge finite value (MFD path
+// counts), which every consumer clamps the same way as inf (overland flow and
+// incision caps, the gelifluction cut-off, the weathering cap)
 __device__ __forceinline__ float2 ff_add(float2 a, float2 b)
 {
     float s = a.x + b.x;
+    if (__builtin_isinf(s)) return make_float2(s, 0.f);
     float v = s - a.x;
     float e = (a.x - (s - v)) + (b.x - v);
     e = e + (a.y + b.y);

@@ -140,6 +140,10 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         if (iv > sT) {
             T softrockinc = sT;
             T bedrockinc = ((T)1.0 - (softrockinc / iv)) * (T)0.0001 * mpow(f_temp, (T)p.Flow_B) * (cslope) * ex;
+            // FP32 form: a discharge past the float range makes the power
+            // inf, and a zero slope must still give a zero term (the FP64
+            // form's power stays finite there)
+            if (sizeof(T) == 4 && cslope == (T)0) bedrockinc = 0;
             iv = softrockinc + bedrockinc;
             if (iv > max_incision) iv = max_incision;
         }
@@ -640,6 +644,42 @@ int lem_fa_phase(lemhip_ctx *c, int iter)
     return c->prec == LEMHIP_FP64 ? fa_t<double>(c) : fa_t<float>(c);
 }
 
+// Runs `launches` (kernel launches on the context stream only: no copies,
+// fills or host waits) directly, or, with graphs on, replays them from a
+// graph captured the first time and again whenever the key (the launch
+// shapes: band of rows, slot count, reduction split) changes
+template <class F>
+static int run_segment(lemhip_ctx *c, GraphSeg &g, long key, F launches)
+{
+    if (!c->use_graphs) return launches();
+    if (g.key != key || !g.exec) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        g.exec = nullptr;
+        g.graph = nullptr;
+        g.key = -1;
+        LEM_CHECK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        const int rc = launches();
+        c->capturing = false;
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->s, &gr);
+        if (rc != LEMHIP_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess || !gr) {
+            c->err = std::string("graph capture: ") + hipGetErrorString(e);
+            return LEMHIP_E_HIP;
+        }
+        g.graph = gr;
+        LEM_CHECK(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        g.key = key;
+    }
+    LEM_CHECK(c, hipGraphLaunch(g.exec, c->s));
+    return LEMHIP_OK;
+}
+
 template <typename T>
 static int mod_t(lemhip_ctx *c)
 {
@@ -647,21 +687,24 @@ static int mod_t(lemhip_ctx *c)
     long N = c->N;
     int ns = c->nslot;
     int g = grid1d(N, 256, 16384);
-    prof_begin(c, PROF_MOD1);
-    k_mod1<T><<<g, 256, 0, c->s>>>(N, c->NP, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa, (const T *)c->fines,
-                                   (const T *)c->nut, (const T *)c->stone, (const T *)c->TotB, c->soilT,
-                                   (const T *)c->temp, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci, (T *)c->geli,
-                                   (T *)c->depo);
-    prof_end(c, PROF_MOD1);
-    LEM_LAUNCHED(c);
-    prof_begin(c, PROF_REDUCE);
-    LEM_TRY(slot_sum<T>(c, c->ero, DS_OFF(totE), c->inci, DS_OFF(totI), c->geli, DS_OFF(totG)));
-    prof_end(c, PROF_REDUCE, 2);
+    const long key = ((N * 256 + ns) * 2048) + c->slot_kbmax;
+    // live timing: the kernels alone when launched directly; the whole
+    // segment (kernel and totals) when replayed from a graph
+    if (c->use_graphs) prof_begin(c, PROF_MOD1);
+    LEM_TRY(run_segment(c, c->gseg[0], key, [&]() -> int {
+        if (!c->use_graphs) prof_begin(c, PROF_MOD1);
+        k_mod1<T><<<g, 256, 0, c->s>>>(N, c->NP, c->p, c->mask, (const T *)c->slopes, (const FS *)c->fa,
+                                       (const T *)c->fines, (const T *)c->nut, (const T *)c->stone,
+                                       (const T *)c->TotB, c->soilT, (const T *)c->temp, (T *)c->slopeptr,
+                                       (T *)c->ero, (T *)c->inci, (T *)c->geli, (T *)c->depo);
+        if (!c->use_graphs) prof_end(c, PROF_MOD1);
+        LEM_LAUNCHED(c);
+        return slot_sum<T>(c, c->ero, DS_OFF(totE), c->inci, DS_OFF(totI), c->geli, DS_OFF(totG));
+    }));
+    if (c->use_graphs) prof_end(c, PROF_MOD1);
     prof_begin(c, PROF_SED);
     LEM_TRY(lem_sed_dag<T>(c));
     prof_end(c, PROF_SED, c->dag_rounds_last);
-    LEM_TRY(slot_sum<T>(c, c->depo, DS_OFF(totD)));
-    k_slots_base<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
     // the low-soil variant sits one allocated grid after the other, whatever
     // band of rows this step covers (an atlas steps only its used rows, and
     // that band grows and shrinks as DEMs come and go)
@@ -671,16 +714,22 @@ static int mod_t(lemhip_ctx *c)
         LEM_LAUNCHED(c);
         c->tcache_ok = true;
     }
-    prof_begin(c, PROF_MOD2);
-    k_mod2<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const FS *)c->fa, (const T *)c->ero, (const T *)c->inci,
-                                   (const T *)c->geli, (const T *)c->depo, (const T *)c->temp, (const T *)c->rain,
-                                   (const T *)c->last_temp, c->soilT, (T *)c->stone, (T *)c->fines, (T *)c->soilM,
-                                   (T *)c->soilB, (T *)c->nut, (T *)c->TotB, c->dem, (T *)c->dz, (T *)c->wC,
-                                   (T *)c->wP, c->demid, c->dsc, ca_hi, ca_lo);
-    prof_end(c, PROF_MOD2);
-    k_slots_post<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
-    LEM_LAUNCHED(c);
-    LEM_TRY(slot_sum<T>(c, c->wC, DS_OFF(totwC), c->wP, DS_OFF(totwP), c->TotB, DS_OFF(totbio)));
+    if (c->use_graphs) prof_begin(c, PROF_MOD2);
+    LEM_TRY(run_segment(c, c->gseg[1], key, [&]() -> int {
+        LEM_TRY(slot_sum<T>(c, c->depo, DS_OFF(totD)));
+        k_slots_base<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
+        if (!c->use_graphs) prof_begin(c, PROF_MOD2);
+        k_mod2<T><<<g, 256, 0, c->s>>>(N, c->p, c->mask, (const FS *)c->fa, (const T *)c->ero, (const T *)c->inci,
+                                       (const T *)c->geli, (const T *)c->depo, (const T *)c->temp,
+                                       (const T *)c->rain, (const T *)c->last_temp, c->soilT, (T *)c->stone,
+                                       (T *)c->fines, (T *)c->soilM, (T *)c->soilB, (T *)c->nut, (T *)c->TotB,
+                                       c->dem, (T *)c->dz, (T *)c->wC, (T *)c->wP, c->demid, c->dsc, ca_hi, ca_lo);
+        if (!c->use_graphs) prof_end(c, PROF_MOD2);
+        k_slots_post<<<(ns + 255) / 256, 256, 0, c->s>>>(c->dem, c->ddesc, ns, c->dsc);
+        LEM_LAUNCHED(c);
+        return slot_sum<T>(c, c->wC, DS_OFF(totwC), c->wP, DS_OFF(totwP), c->TotB, DS_OFF(totbio));
+    }));
+    if (c->use_graphs) prof_end(c, PROF_MOD2);
     return LEMHIP_OK;
 }
 
