@@ -188,7 +188,8 @@ int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const in
  * (64x16, 32x32, 16x64, 128x8, 64x32 or 32x64) and checks convergence on the
  * host every dag_batch round launches; plateau routing advances
  * plateau_rounds_per_launch (4, 8 or 16) synchronous rounds per launch.
- * Results never depend on these choices.  lemhip_autotune makes the next step
+ * Results never depend on these choices.  lemhip_set_tuning cancels a
+ * pending autotune.  lemhip_autotune makes the next step
  * time every candidate on the context's own data (the timed passes are
  * idempotent), keep the fastest, and record the times in tuned_ms:
  * [0..5] flow accumulation per tile shape, [8..10] plateau routing per rounds

@@ -895,6 +895,9 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     if (t->fp32_integer_fa != 0 && t->fp32_integer_fa != 1) { c->err = "fp32_integer_fa must be 0 or 1"; return LEMHIP_E_ARG; }
     if (t->use_graphs != 0 && t->use_graphs != 1) { c->err = "use_graphs must be 0 or 1"; return LEMHIP_E_ARG; }
     c->use_graphs = t->use_graphs != 0;
+    // an explicit choice replaces a pending autotune
+    c->tune_dag_pending = false;
+    c->tune_plat_pending = false;
     c->dag_shape = shape;
     c->dag_batch = t->dag_batch;
     c->plat_k = k;

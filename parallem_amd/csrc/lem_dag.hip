@@ -923,6 +923,7 @@ int lem_tune_dag(lemhip_ctx *c)
         if (ms < best) { best = ms; bb = BATCHES[q]; }
     }
     c->dag_batch = bb;
+    c->tune_dag_pending = false;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return LEMHIP_OK;

@@ -1436,6 +1436,7 @@ static int tune_plateau(lemhip_ctx *c, const double *zs, TS *slopes)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->plat_k = bestk;
+    c->tune_plat_pending = false;
     return LEMHIP_OK;
 }
 

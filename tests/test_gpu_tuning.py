@@ -51,3 +51,26 @@ def test_bad_tuning_is_rejected():
     with pytest.raises(lemhip.LemError):
         ctx.set_tuning(plateau_rounds_per_launch=5)
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [lemhip.FP64, lemhip.FP32_KAHAN])
+def test_autotune_runs_once(precision):
+    """lemhip_autotune tunes on the next step only: a shape set afterwards is
+    kept by the following steps and the phase-level calls."""
+    dem = synth.fbm_512(side=200, seed_offset=33)
+    H, W = dem.z.shape
+    ctx = lemhip.LemContext(lemhip.default_params(W, H, 0), precision, 0)
+    ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+    ctx.autotune()
+    ctx.step(-1000)
+    first = ctx.tuning()
+    assert all(x > 0 for x in first["tuned_ms"][:6])
+    other = (16, 64) if tuple(first["dag_tile"]) != (16, 64) else (64, 16)
+    ctx.set_tuning(dag_tile=other, plateau_rounds_per_launch=4)
+    ctx.step(-999)
+    ctx.flow_direction(-998)
+    ctx.runoff()
+    ctx.flow_accum(-998)
+    t = ctx.tuning()
+    assert tuple(t["dag_tile"]) == other and t["plateau_rounds_per_launch"] == 4
+    ctx.close()
