@@ -31,6 +31,7 @@ extern "C" {
 #define LEMHIP_E_NOMEM (-3)
 #define LEMHIP_E_NOT_CONVERGED (-4)
 #define LEMHIP_E_STATE (-5)
+#define LEMHIP_E_SIZE (-6)   /* input larger than a fixed capacity */
 
 /* Precision forms. */
 #define LEMHIP_FP64 0        /* reference dtypes, bit-exact with the FP64 restatement */
@@ -232,6 +233,17 @@ int lemhip_profile_read(lemhip_ctx *ctx, int n, double *ms, long long *launches)
 /* Test hook with no counterpart in the reference: evaluates the device FP64
  * math routines (fn 0 exp, 1 log, 2 log10, 3 pow, 4 tanh, 5 expm1) on n inputs. */
 int lemhip_math_eval(int fn, int n, const double *x, const double *y, double *out);
+
+/* The same evaluation without any allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Inputs and output
+ * are host arrays as above; the device staging is a fixed array of
+ * 3 * LEMHIP_NOALLOC_MAX doubles per device, so n above LEMHIP_NOALLOC_MAX
+ * returns LEMHIP_E_SIZE and nothing is run.  Every call on a device shares
+ * that device's staging: calls that overlap on one device (from several host
+ * threads) race. */
+#define LEMHIP_NOALLOC_MAX 65536
+int lemhip_math_eval_noalloc(int fn, int n, const double *x, const double *y, double *out);
 
 /* Device properties for measurement code (no reference counterpart). */
 int lemhip_device_info(int device_id, char *name, int name_len, int *cu_count, double *mem_gb);

@@ -8,6 +8,7 @@
 #include <vector>
 #include <type_traits>
 #include <thread>
+#include <mutex>
 
 static int grid1d(long n, int block = 256, int cap = 8192)
 {
@@ -1277,6 +1278,51 @@ extern "C" int lemhip_bytes_per_cell(int precision, int atlas, double *fixed_per
     // two (at most 16 per label); per-label arrays 21 B with 25% headroom;
     // for at most one sink label per 16 cells
     if (grow_per_cell) *grow_per_cell = (16 * 64.0 + 1.25 * 21.0) / 16.0;
+    return LEMHIP_OK;
+}
+
+// ------------------------------------------------ allocation-free math hook
+
+// Device staging of lemhip_math_eval_noalloc: x, y and the result, at most
+// LEMHIP_NOALLOC_MAX doubles each (3 * 65536 * 8 B = 1.5 MiB on every device
+// the library runs on, held until the process ends).  Host code never uses
+// the array's name as an address: each device's copy is looked up once with
+// hipGetSymbolAddress and kept in noalloc_ptr.
+static __device__ __align__(256) double scratch[3 * LEMHIP_NOALLOC_MAX];
+#define NOALLOC_DEVICES 64
+static std::mutex noalloc_mu;
+static double *noalloc_ptr[NOALLOC_DEVICES];
+
+// the current device's copy of `scratch` (set up on the first call there)
+static int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_DEVICES) return LEMHIP_E_HIP;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LEMHIP_E_HIP;
+        noalloc_ptr[dev] = (double *)p;
+    }
+    *out = noalloc_ptr[dev];
+    return LEMHIP_OK;
+}
+
+extern "C" int lemhip_math_eval_noalloc(int fn, int n, const double *x, const double *y, double *out)
+{
+    if (!x || !y || !out || n <= 0) return LEMHIP_E_ARG;
+    if (n > LEMHIP_NOALLOC_MAX) return LEMHIP_E_SIZE;
+    double *d = nullptr;
+    LEM_TRY(noalloc_scratch(&d));
+    // every element of the staging is written before it is read: nothing
+    // carried over from an earlier call is used
+    if (hipMemcpy(d, x, (size_t)n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, y, (size_t)n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return LEMHIP_E_HIP;
+    k_math_eval<<<(n + 255) / 256, 256>>>(fn, n, d, d + n, d + 2 * n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, d + 2 * n, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return LEMHIP_E_HIP;
     return LEMHIP_OK;
 }
 
