@@ -45,19 +45,6 @@ __device__ __forceinline__ int fx_dir(uint8_t code)
     }
 }
 
-// receiver of cell (r, col): a mask cell in direction fd, or -1 (no code,
-// off the grid, nodata, or the outlet whose link is cut)
-__device__ __forceinline__ long fx_recv(int W, int H, int r, int col, long i, const uint8_t *mask, const uint8_t *fd,
-                                        int outlet)
-{
-    const int d = fx_dir(fd[i]);
-    if (d < 0 || i == outlet) return -1;
-    const int nr = r + cdy(d), nc = col + cdx(d);
-    if (nr < 0 || nr >= H || nc < 0 || nc >= W) return -1;
-    const long n = (long)nr * W + nc;
-    return mask[n] ? n : -1;
-}
-
 __device__ __forceinline__ unsigned long long fx_quant(float w, int shift)
 {
     return (unsigned long long)llrint(ldexp((double)w, shift));
@@ -99,70 +86,131 @@ struct FxArgs {
     DemScal *dsw;
 };
 
+// n items of this thread appended to a global list with one atomic per
+// block; returns this thread's first index (every thread must call it)
+__device__ __forceinline__ int fx_block_append(int *counter, int n, int *s_wave, int *s_base)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            const int cnt = s_wave[w];
+            s_wave[w] = tot;
+            tot += cnt;
+        }
+        *s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const int pos = *s_base + s_wave[wave] + x - n;
+    __syncthreads();
+    return pos;
+}
+
+// padded (TW+2) x (TH+2) index of tile cell (lr, lc); the ring holds the
+// neighbours in other tiles (or off the grid)
+template <int TW> __device__ __forceinline__ int fx_pad(int lr, int lc) { return (lr + 1) * (TW + 2) + lc + 1; }
+
 template <int TW, int TH, bool FINAL>
 __global__ void __launch_bounds__(256) k_fx_tile(FxArgs a)
 {
-    constexpr int NC = TW * TH, PER = NC / 256;
+    constexpr int NC = TW * TH, PER = NC / 256, PW = TW + 2, NP = (TW + 2) * (TH + 2);
+    constexpr int NPPER = (NP + 255) / 256;
     static_assert(NC % 256 == 0, "tile shape");
     __shared__ unsigned long long x[NC];
     __shared__ uint16_t J[NC];
     __shared__ uint16_t R[FINAL ? 1 : NC];
+    __shared__ uint8_t sfd[NP], smk[NP];
+    __shared__ uint8_t isexit[FINAL ? 1 : NC];
+    __shared__ int s_wave[4], s_base;
     const int W = a.W, H = a.H;
     const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH;
     for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
         const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
-        long ext[PER];   // receiver outside the tile (an exit), or -1
-        int slot[PER];
-        bool inb[PER];
         __syncthreads();
+        // directions and mask of the tile and its ring, every load in flight
+        // before the first store (off-grid positions: no mask, no code)
+        {
+            uint8_t vf[NPPER], vm[NPPER];
+#pragma unroll
+            for (int q = 0; q < NPPER; q++) {
+                const int k = threadIdx.x + q * 256;
+                const int gy = ty0 - 1 + k / PW, gx = tx0 - 1 + k % PW;
+                const bool in = k < NP && gy >= 0 && gy < H && gx >= 0 && gx < W;
+                const long n = (long)gy * W + gx;
+                vf[q] = in ? a.fd[n] : 0;
+                vm[q] = in ? a.mask[n] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < NPPER; q++) {
+                const int k = threadIdx.x + q * 256;
+                if (k < NP) {
+                    sfd[k] = vf[q];
+                    smk[k] = vm[q];
+                }
+            }
+        }
+        __syncthreads();
+        int nexit = 0;
+        uint32_t exitbits = 0;
+        int slot[PER];
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int li = threadIdx.x + q * 256;
-            const int r = ty0 + li / TW, col = tx0 + li % TW;
-            ext[q] = -1;
+            const int lr = li / TW, lc = li % TW;
+            const int r = ty0 + lr, col = tx0 + lc;
+            const int p = fx_pad<TW>(lr, lc);
             slot[q] = 0;
-            inb[q] = r < H && col < W;
             unsigned long long v = 0;
             uint16_t j = FX_NONE;
-            if (inb[q]) {
+            bool ex = false;
+            if (smk[p]) {
                 const long i = (long)r * W + col;
-                if (a.mask[i]) {
-                    const int s = a.demid ? (int)a.demid[i] : 0;
-                    slot[q] = s;
-                    const int outlet = a.ds[s].outlet;
-                    const int sh = a.shift[s];
-                    v = fx_quant(a.w[i], sh);
-                    if (fx_tagged(i, W, a.mask, outlet)) v += FX_TAG;
-                    const long n = fx_recv(W, H, r, col, i, a.mask, a.fd, outlet);
-                    if (n >= 0) {
-                        const int nr = (int)(n / W) - ty0, nc = (int)(n % W) - tx0;
+                const int s = a.demid ? (int)a.demid[i] : 0;
+                slot[q] = s;
+                const int outlet = a.ds[s].outlet;
+                v = fx_quant(a.w[i], a.shift[s]);
+                if (fx_tagged(i, W, a.mask, outlet)) v += FX_TAG;
+                const int d = fx_dir(sfd[p]);
+                if (d >= 0 && i != outlet) {
+                    const int pn = p + cdy(d) * PW + cdx(d);
+                    if (smk[pn]) {
+                        const int nr = lr + cdy(d), nc = lc + cdx(d);
                         if (nr >= 0 && nr < TH && nc >= 0 && nc < TW) j = (uint16_t)(nr * TW + nc);
-                        else ext[q] = n;
+                        else ex = true;
                     }
-                    if (FINAL) {
-                        // exits of other tiles that drain into this cell (only
-                        // tile border cells can have them)
-                        const int lr = li / TW, lc = li % TW;
-                        if (lr == 0 || lr == TH - 1 || lc == 0 || lc == TW - 1) {
+                }
+                if (FINAL && (lr == 0 || lr == TH - 1 || lc == 0 || lc == TW - 1)) {
+                    // exits of other tiles that drain into this cell
 #pragma unroll
-                            for (int d = 0; d < 8; d++) {
-                                const int yr = r + cdy(d), yc = col + cdx(d);
-                                const int tr = lr + cdy(d), tc = lc + cdx(d);
-                                if (tr >= 0 && tr < TH && tc >= 0 && tc < TW) continue;   // inside the tile
-                                if (yr < 0 || yr >= H || yc < 0 || yc >= W) continue;
-                                const long y = (long)yr * W + yc;
-                                if (!a.mask[y] || a.fd[y] != donor_code(d)) continue;
-                                const int sy = a.demid ? (int)a.demid[y] : 0;
-                                if (y == a.ds[sy].outlet) continue;                       // cut link
-                                v += a.ex_val[a.exit_id[y]];
-                            }
-                        }
+                    for (int dd = 0; dd < 8; dd++) {
+                        const int tr = lr + cdy(dd), tc = lc + cdx(dd);
+                        if (tr >= 0 && tr < TH && tc >= 0 && tc < TW) continue;   // inside the tile
+                        const int pn = p + cdy(dd) * PW + cdx(dd);
+                        if (!smk[pn] || sfd[pn] != donor_code(dd)) continue;
+                        const long y = (long)(r + cdy(dd)) * W + (col + cdx(dd));
+                        if (y == outlet) continue;                                // cut link
+                        v += a.ex_val[a.exit_id[y]];
                     }
                 }
             }
             x[li] = v;
             J[li] = j;
-            if (!FINAL) R[li] = j == FX_NONE ? (uint16_t)li : j;
+            if (!FINAL) {
+                R[li] = j == FX_NONE ? (uint16_t)li : j;
+                isexit[li] = ex ? 1 : 0;
+            }
+            if (ex) {
+                nexit++;
+                exitbits |= 1u << q;
+            }
         }
         __syncthreads();
         // pointer doubling: after round t every cell holds the sum over the
@@ -194,41 +242,42 @@ __global__ void __launch_bounds__(256) k_fx_tile(FxArgs a)
             if (!__syncthreads_or(act)) break;
         }
         // outputs
+        if (!FINAL) {
+            int k = fx_block_append(a.ex_count, nexit, s_wave, &s_base);
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            if (!inb[q]) continue;
-            const int li = threadIdx.x + q * 256;
-            const int lr = li / TW, lc = li % TW;
-            const long i = (long)(ty0 + lr) * W + (tx0 + lc);
-            const unsigned long long s = x[li];
-            if (!FINAL) {
-                const bool border = lr == 0 || lr == TH - 1 || lc == 0 || lc == TW - 1;
-                if (ext[q] >= 0) {
-                    const int k = atomicAdd(a.ex_count, 1);
+            for (int q = 0; q < PER; q++) {
+                const int li = threadIdx.x + q * 256;
+                const int lr = li / TW, lc = li % TW;
+                const int r = ty0 + lr, col = tx0 + lc;
+                if (r >= H || col >= W) continue;
+                const long i = (long)r * W + col;
+                if ((exitbits >> q) & 1) {
                     if (k < a.ex_cap) {
+                        const int d = fx_dir(sfd[fx_pad<TW>(lr, lc)]);
                         a.exit_id[i] = k;
-                        a.ex_val[k] = s;
-                        a.ex_recv[k] = (int)ext[q];
+                        a.ex_val[k] = x[li];
+                        a.ex_recv[k] = (int)((long)(r + cdy(d)) * W + (col + cdx(d)));
                     }
+                    k++;
                 }
-                if (border) {
-                    // the in-tile root is an exit exactly when it has a receiver
-                    // outside the tile: read back from the root's own state
+                if (lr == 0 || lr == TH - 1 || lc == 0 || lc == TW - 1) {
                     const int root = R[li];
-                    const int rr = ty0 + root / TW, rc = tx0 + root % TW;
-                    const long ri = (long)rr * W + rc;
-                    long rn = -1;
-                    if (a.mask[ri]) {
-                        const int s2 = a.demid ? (int)a.demid[ri] : 0;
-                        rn = fx_recv(W, H, rr, rc, ri, a.mask, a.fd, a.ds[s2].outlet);
-                    }
-                    a.xroot[i] = rn >= 0 ? (int)ri : -1;
+                    a.xroot[i] = isexit[root] ? (int)((long)(ty0 + root / TW) * W + (tx0 + root % TW)) : -1;
                 }
-            } else {
-                if (!a.mask[i]) {
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int li = threadIdx.x + q * 256;
+                const int lr = li / TW, lc = li % TW;
+                const int r = ty0 + lr, col = tx0 + lc;
+                if (r >= H || col >= W) continue;
+                const long i = (long)r * W + col;
+                if (!smk[fx_pad<TW>(lr, lc)]) {
                     a.fa[i] = make_float2(0.f, 0.f);
                     continue;
                 }
+                const unsigned long long s = x[li];
                 const int s2 = slot[q];
                 if (i == a.ds[s2].outlet) a.dsw[s2].fx_outlet_sum = s;
                 if (s >= FX_TAG) {
