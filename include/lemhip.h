@@ -14,7 +14,9 @@
  * Conventions: every call returns 0 on success or a negative LEMHIP_E_* code
  * (never exits the process); lemhip_last_error() describes the last failure.
  * One context per host thread; work runs on the context's own HIP stream and
- * every call is synchronous at the boundary.
+ * every call that fills a host buffer or returns results is synchronous at
+ * the boundary (lemhip_slot_set_climate only enqueues its kernel: every later
+ * call on the context sees its fields).
  */
 #ifndef LEMHIP_H
 #define LEMHIP_H

@@ -30,8 +30,9 @@ __global__ void k_init_rect(int W, DemDesc D, int slot, lem_params p, const doub
                             uint8_t *mask, double *soilT, T *rain, T *temp, T *last_rain, T *last_temp, T *soilM,
                             T *stone, T *fines, T *nut, T *soilB, T *TotB, T *w, T *slopeptr, T *ero, T *inci,
                             T *geli, T *depo, T *dz, T *wC, T *wP, uint8_t *fd, uint8_t *SFD, T *slopes, FS *fa,
-                            long N)
+                            long N, int *soilneg)
 {
+    int neg = 0;
     double rainmax = p.rain * 2;                     // Data.cpp:246-250
     double rainmin = p.rain;
     double increment = (rainmax - rainmin) / D.h;
@@ -47,6 +48,7 @@ __global__ void k_init_rect(int W, DemDesc D, int slot, lem_params p, const doub
         bool m = !(z == -9999);                      // memory.cu:229-237
         mask[x] = m ? 1 : 0;
         double sT = (z > 0) ? ((z - 400) / 1400) * 5 : 0.0; // memory.cu:207-208
+        neg += (m && sT < 0.0) ? 1 : 0;
         if (soilT_mode == 1 && sT < 0.0) sT = 0.0;
         double r, t;
         if (m) {
@@ -72,6 +74,11 @@ __global__ void k_init_rect(int W, DemDesc D, int slot, lem_params p, const doub
         ero[x] = 0; geli[x] = 0; inci[x] = 0; depo[x] = 0; wC[x] = 0; wP[x] = 0;
         fa[x] = FS{};
         for (int d = 0; d < 8; d++) slopes[d * N + x] = 0;   // Data.cpp:181-185
+    }
+    if (soilneg) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) neg += __shfl_down(neg, o);
+        if ((threadIdx.x & 63) == 0 && neg) atomicAdd(soilneg, neg);
     }
 }
 
@@ -363,14 +370,15 @@ extern "C" int lemhip_create_atlas(const lem_params *p, int atlas_w, int atlas_h
             (double *)c->fines, (double *)c->nut, (double *)c->soilB, (double *)c->TotB, (double *)c->w,
             (double *)c->slopeptr, (double *)c->ero, (double *)c->inci, (double *)c->geli, (double *)c->depo,
             (double *)c->dz, (double *)c->wC, (double *)c->wP, c->fd, c->SFD, (double *)c->slopes, (double *)c->fa,
-            c->N);
+            c->N, nullptr);
     else
         k_init_rect<float, float2><<<grid1d(c->N), 256, 0, c->s>>>(
             c->W, all, DEM_NONE, c->p, nullptr, nullptr, 0, c->demid, c->dem, c->mask, c->soilT, (float *)c->rain,
             (float *)c->temp, (float *)c->last_rain, (float *)c->last_temp, (float *)c->soilM, (float *)c->stone,
             (float *)c->fines, (float *)c->nut, (float *)c->soilB, (float *)c->TotB, (float *)c->w,
             (float *)c->slopeptr, (float *)c->ero, (float *)c->inci, (float *)c->geli, (float *)c->depo,
-            (float *)c->dz, (float *)c->wC, (float *)c->wP, c->fd, c->SFD, (float *)c->slopes, (float2 *)c->fa, c->N);
+            (float *)c->dz, (float *)c->wC, (float *)c->wP, c->fd, c->SFD, (float *)c->slopes, (float2 *)c->fa, c->N,
+            nullptr);
     LEM_LAUNCHED(c);
     return upload_desc(c);
 }
@@ -430,28 +438,22 @@ static void launch_init_rect(lemhip_ctx *c, const DemDesc &D, int slot, const do
         (T *)c->temp, (T *)c->last_rain, (T *)c->last_temp, (T *)c->soilM, (T *)c->stone, (T *)c->fines,
         (T *)c->nut, (T *)c->soilB, (T *)c->TotB, (T *)c->w, (T *)c->slopeptr, (T *)c->ero, (T *)c->inci,
         (T *)c->geli, (T *)c->depo, (T *)c->dz, (T *)c->wC, (T *)c->wP, c->fd, c->SFD, (T *)c->slopes, (FS *)c->fa,
-        c->NP);
+        c->NP, slot == DEM_NONE ? nullptr : &c->dsc[slot].soilT_negative);
 }
 
-// Input diagnostics of one DEM (host layout w x h): valid cells the
-// reference's label kernels also reach from threads past a row end
-// (newflow.cu:358-360: the block of 16 threads covering the last columns
-// continues into the next row's first P columns), and valid cells with a
-// negative formula soil thickness (memory.cu:207-208)
-static void input_diagnostics(const double *dem, int w, int h, int *phantom, int *soilneg)
+// Valid cells of one DEM (host layout w x h) that the reference's label
+// kernels also reach from threads past a row end (newflow.cu:358-360: the
+// block of 16 threads covering the last columns continues into the next
+// row's first P columns).  The count of valid cells with a negative formula
+// soil thickness (memory.cu:207-208) is taken by the initialisation kernel.
+static int phantom_cells(const double *dem, int w, int h)
 {
     const int P = (w / 16 + 1) * 16 - w;
-    int ph = 0, sn = 0;
+    int ph = 0;
     for (int r = 1; r < h; r++)
         for (int q = 0; q < P && q < w; q++)
             if (dem[(long)r * w + q] != -9999.0) ph++;
-    const long n = (long)w * h;
-    for (long i = 0; i < n; i++) {
-        const double z = dem[i];
-        if (z != -9999.0 && z > 0 && z < 400) sn++;
-    }
-    *phantom = ph;
-    *soilneg = sn;
+    return ph;
 }
 
 // Data.cpp:38-66 (outlet), memory.cu:200-240 and Data.cpp:120-271 (initial
@@ -461,17 +463,16 @@ static int init_slot(lemhip_ctx *c, int slot, const double *dem, const double *r
     c->tcache_ok = false;   // the slot's temperatures are set
     DemDesc D;
     LEM_TRY(slot_rect(c, slot, 1, &D));
-    input_diagnostics(dem, D.w, D.h, &c->hphantom[slot], &c->hsoilneg[slot]);
+    c->hphantom[slot] = phantom_cells(dem, D.w, D.h);
     long n = (long)D.w * D.h;
     LEM_TRY(dstage_reserve(c, (rain ? 2 : 1) * n * 8));
     double *dem_d = (double *)c->dstage;
     double *rain_d = rain ? dem_d + n : nullptr;
     LEM_CHECK(c, hipMemcpyAsync(dem_d, dem, n * 8, hipMemcpyHostToDevice, c->s));
     if (rain) LEM_CHECK(c, hipMemcpyAsync(rain_d, rain, n * 8, hipMemcpyHostToDevice, c->s));
+    k_slot_scal_init<<<1, 1, 0, c->s>>>(c->dsc, slot);
     if (c->prec == LEMHIP_FP64) launch_init_rect<double>(c, D, slot, dem_d, rain_d, soilT_mode);
     else launch_init_rect<float>(c, D, slot, dem_d, rain_d, soilT_mode);
-    LEM_LAUNCHED(c);
-    k_slot_scal_init<<<1, 1, 0, c->s>>>(c->dsc, slot);
     LEM_LAUNCHED(c);
     c->hdesc[slot] = D;
     c->hstate[slot] = 2;
@@ -1025,6 +1026,8 @@ extern "C" int lemhip_set_climate(lemhip_ctx *c, double rain, double temp)
     return LEMHIP_OK;
 }
 
+// no host wait: the new fields are written by a kernel on the context's
+// stream, ahead of whatever the next call runs there
 extern "C" int lemhip_slot_set_climate(lemhip_ctx *c, int slot, double rain, double temp)
 {
     if (!c) return LEMHIP_E_ARG;
@@ -1032,7 +1035,6 @@ extern "C" int lemhip_slot_set_climate(lemhip_ctx *c, int slot, double rain, dou
     LEM_TRY(slot_rect(c, slot, 2, &D));
     LEM_CHECK(c, hipSetDevice(c->dev));
     LEM_TRY(set_climate_rect(c, D, rain, temp));
-    LEM_CHECK(c, hipStreamSynchronize(c->s));
     return LEMHIP_OK;
 }
 
@@ -1073,7 +1075,7 @@ extern "C" int lemhip_slot_summary(lemhip_ctx *c, int slot, lem_summary *o)
     o->sed_uncomputed = c->h_sed_unc;
     o->totwater = h.totwater;
     o->phantom_cells = c->hphantom[slot];
-    o->soilT_negative = c->hsoilneg[slot];
+    o->soilT_negative = h.soilT_negative;
     return LEMHIP_OK;
 }
 

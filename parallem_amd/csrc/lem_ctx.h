@@ -70,15 +70,12 @@ struct lemhip_ctx {
     DemDesc hdesc[LEM_MAX_SLOTS + 1] = {};   // active rectangles (w == 0: inactive); entry 255 stays empty
     DemDesc hrect[LEM_MAX_SLOTS] = {};       // reserved rectangles
     uint8_t hstate[LEM_MAX_SLOTS] = {};      // 0 free, 1 reserved, 2 active
-    // per-slot input diagnostics, from the host DEM at initialisation:
+    // per-slot input diagnostic, from the host DEM at initialisation:
     //  phantom: valid cells in rows 1..h-1 of columns < P = (w/16 + 1)*16 - w,
     //           which the reference's label kernels also visit from threads
     //           past the row end (newflow.cu:358-360, 434-436); 0 means every
     //           legal schedule of those kernels equals the bounds-checked one
-    //  soilT_neg: valid cells whose formula soil thickness is negative
-    //           (0 < dem < 400, memory.cu:207-208)
     int hphantom[LEM_MAX_SLOTS] = {};
-    int hsoilneg[LEM_MAX_SLOTS] = {};
     bool atlas = false;
     ShelfPack pack;
     DemScal *dsc = nullptr;       // device per-slot scalars (LEM_MAX_SLOTS + 1 entries)

@@ -64,6 +64,7 @@ struct DemScal {
     int zero_seen;
     int first_iter;       // this step is the slot's first iteration (mfd_simple.cu:336)
     unsigned long long fx_outlet_sum;   // the outlet's integer flow sum (FP32 single-direction accumulation)
+    int soilT_negative;   // valid cells with a negative formula soil thickness at initialisation
 };
 
 // Per-slot reductions split a slot's rows over slot_kb(h) blocks (row k goes

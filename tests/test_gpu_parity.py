@@ -100,6 +100,15 @@ def test_reference_soilT_formula_nan_path():
     conc_erosion through 0/0 and NaN; the NaN pattern must match too."""
     dem = synth.fbm_512(side=96, seed_offset=3)
     ctx, orc = _pair(dem, soilT_mode="formula")
+    # input diagnostics: valid cells with 0 < z < 400 (negative formula soil
+    # thickness), and valid cells in columns 0..P-1 of rows 1.. (P = 16 - w % 16)
+    z = dem.z
+    valid = z != -9999.0
+    s = ctx.summary()
+    assert s["soilT_negative"] == int((valid & (z > 0) & (z < 400)).sum()) > 0
+    w = z.shape[1]
+    P = min((w // 16 + 1) * 16 - w, w)
+    assert s["phantom_cells"] == int(valid[1:, :P].sum())
     for k in range(3):
         ctx.step(-1000 + k)
         orc.step(-1000 + k)
