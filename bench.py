@@ -8,8 +8,8 @@ unclamped formula turns every cell below 400 m into NaN after one step):
 * value — batched FP32 + compensated accumulation: B (default 8) independent
   4096 x 4096 fBm DEMs (diamond-square H = 0.8, 0-800 m, 0.5 % tilt, 400 pits,
   100 plateaus, 5 % nodata frame, per-cell lognormal(0, 0.3) rain rescaled to
-  a 1000 mm/yr mean; seeds differ per DEM) packed in one atlas grid and
-  stepped together;
+  a 1000 mm/yr mean; seeds differ per DEM) split over --groups (default 2)
+  atlas grids, stepped together on one stream each;
 * single_fp32 — one such DEM alone, same precision form;
 * plain_fp64 — one such DEM alone in the plain FP64 form (bit-exact with the
   CPU restatement): the reference's own arithmetic.
@@ -64,6 +64,11 @@ def kernel_bytes(kernel, t):
                soilT 8 read; SlopePtr, ero, inci, geli, depo t each written.
     flowdirs   (k_flowdirs): dem 8 (stencil, read once from HBM), mask 1 read;
                fd, SFD 1 each + downhill Slopes entries (at most 8t) written.
+    plat_rounds (k_plat_block, plateau routing on the raised surface): surface
+               8, mask 1, equal-height bits 1 read; fd, SFD 1 each and the
+               distance 4 written, once per cell.
+    spill_relax (k_minimax_relax_reached) is counted per basin graph, not per
+               cell: see graph_bytes.
     """
     return {
         "fa_rounds": 2 + t + 8,
@@ -71,16 +76,28 @@ def kernel_bytes(kernel, t):
         "mod2": 8 + 15 * t + 8 + 8 + 9 * t + 8 + 8,
         "mod1": 8 * t + 8 + 5 * t + 8 + 5 * t,
         "flowdirs": 8 + 1 + 2 + 8 * t,
+        "plat_rounds": 8 + 1 + 1 + 2 + 4,
     }.get(kernel)
 
 
+def graph_bytes(nedges, nlabels):
+    """Algorithmic bytes of the spill-height pass: the basin-pair edge list
+    (two labels 4 each, weight 8) read once, each label's spill height 8
+    written once."""
+    return 16 * nedges + 8 * nlabels
+
+
 # kernel-level timing slots (HIP events around exactly these launches)
-KERNEL_SLOTS = ("fa_rounds", "sed_rounds", "mod1", "mod2", "flowdirs")
+KERNEL_SLOTS = ("fa_rounds", "sed_rounds", "mod1", "mod2", "flowdirs", "plat_rounds", "spill_relax")
 # rocprofv3 kernel names of those slots (the round kernels carry the tuned tile shape)
-def rocprof_name(slot, tuning):
+def rocprof_name(slot, tuning, t=4):
     tw, th = tuning["dag_tile"]
+    ts = "float" if t == 4 else "double"
     return {"fa_rounds": f"k_dag_round<{tw}, {th}, FaCell", "sed_rounds": f"k_dag_round<{tw}, {th}, SedCell",
-            "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs"}[slot]
+            "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs",
+            # pass 2 (raised surface, no slope marks) holds nearly all the rounds
+            "plat_rounds": f"k_plat_block<{tuning['plateau_rounds_per_launch']}, {ts}, false>",
+            "spill_relax": "k_minimax_relax_reached"}[slot]
 
 
 def dems(count):
@@ -175,28 +192,35 @@ def run_batched(ds, precision, steps, warmup, tuning=None, groups=1):
     total, lat, prof = timed(grp.step, steps, warmup, grp, autotune=tuning is None)
     summ = [a.summary(s) for a, sl in zip(atlases, slots) for s in sl]
     summ[0]["tuning"] = atlases[0].tuning()
+    # basin graphs of the last step (edge and label counts are per atlas)
+    firsts = [a.summary(sl[0]) for a, sl in zip(atlases, slots)]
+    graph = (sum(f["nedges"] for f in firsts), sum(f["nlabels"] for f in firsts))
     for a in atlases:
         a.close()
-    return total, lat, prof, summ, dims
+    return total, lat, prof, summ, dims, graph
 
 
 def phase_ms(prof, steps):
     return {k: round(v[0] / steps, 3) for k, v in prof.items()}
 
 
-def roofline(prof, steps, cells, t, tuning):
+def roofline(prof, steps, cells, t, tuning, graph):
     """Dominant kernel by live HIP-event time; achieved = algorithmic bytes
     per launch / average launch duration (= bytes of all its launches / their
-    total time)."""
+    total time).  graph: (edges, labels) of the basin graphs of one step."""
     dom = max(KERNEL_SLOTS, key=lambda k: prof[k][0])
     ms, launches = prof[dom]
-    b = kernel_bytes(dom, t)
-    # the round kernels share one pass over the grid per step; the others make
-    # one pass per launch
-    passes = steps if dom in ("fa_rounds", "sed_rounds") else launches
-    total_bytes = b * cells * passes
+    if dom == "spill_relax":
+        total_bytes = graph_bytes(*graph) * steps
+        b = graph_bytes(*graph) / cells
+    else:
+        b = kernel_bytes(dom, t)
+        # the round kernels share one pass over the grid per step; the others
+        # make one pass per launch
+        passes = steps if dom in ("fa_rounds", "sed_rounds", "plat_rounds") else launches
+        total_bytes = b * cells * passes
     ach = total_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
-    return {"kernel": rocprof_name(dom, tuning), "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+    return {"kernel": rocprof_name(dom, tuning, t), "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "achieved": ach, "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None,
             "bytes_per_cell": b, "launches_per_step": launches / steps,
             "avg_launch_us": 1e3 * ms / launches if launches else None,
@@ -295,8 +319,8 @@ def main():
     cells_b = args.batch * SIDE * SIDE
 
     print("[bench] batched FP32+Kahan form", file=sys.stderr, flush=True)
-    tb, latb, profb, summb, atlas_dims = run_batched(ds[:args.batch], lemhip.FP32_KAHAN, args.steps, args.warmup,
-                                                     tuning=fixed, groups=args.groups)
+    tb, latb, profb, summb, atlas_dims, graphb = run_batched(ds[:args.batch], lemhip.FP32_KAHAN, args.steps,
+                                                             args.warmup, tuning=fixed, groups=args.groups)
     value = cells_b * args.steps / tb
 
     single32 = plain = None
@@ -312,7 +336,8 @@ def main():
                  "p50_ms": float(np.percentile(lat64, 50) * 1e3), "p99_ms": float(np.percentile(lat64, 99) * 1e3),
                  "phase_ms_per_step": phase_ms(prof64, args.steps),
                  "step_roofline_frac": SIDE * SIDE * args.steps / t64 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9),
-                 "roofline": roofline(prof64, args.steps, SIDE * SIDE, 8, s64["tuning"]), "tuning": s64["tuning"],
+                 "roofline": roofline(prof64, args.steps, SIDE * SIDE, 8, s64["tuning"],
+                                      (s64["nedges"], s64["nlabels"])), "tuning": s64["tuning"],
                  "last_step": {k: s64[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2",
                                                     "fa_uncomputed", "sed_uncomputed")}}
 
@@ -323,7 +348,7 @@ def main():
               f"{args.cpu_serial_side}^2, 1 step each)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_side, args.cpu_serial_side, threads)
 
-    roof = roofline(profb, args.steps, sum(w * h for w, h in atlas_dims), 4, summb[0]["tuning"])
+    roof = roofline(profb, args.steps, sum(w * h for w, h in atlas_dims), 4, summb[0]["tuning"], graphb)
     roof["traffic"] = measured_traffic(roof["kernel"], summb[0]["tuning"], args.batch)
     if roof["traffic"] is not None:
         roof["traffic_note"] = "HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"

@@ -197,9 +197,9 @@ int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const in
  * idempotent), keep the fastest, and record the times in tuned_ms:
  * [0..5] flow accumulation per tile shape, [8..10] plateau routing per rounds
  * per launch, [11..13] flow accumulation with batches 4, 8, 32.  In the FP32
- * form under single flow directions, flow accumulation itself does not use
- * the engine (fp32_integer_fa below); the tile shape then serves sediment
- * routing. */
+ * form under single flow directions, flow accumulation does not use the
+ * engine by default (fp32_integer_fa below); the tile shape then serves
+ * sediment routing. */
 typedef struct {
     int dag_tile_w, dag_tile_h;
     int dag_batch;
@@ -215,6 +215,15 @@ typedef struct {
      * of rows or the slot count changes); 0 (default): launched directly.
      * Results are identical. */
     int use_graphs;
+    /* FP32 form with single flow directions: 1 routes sediment by
+     * contracting the donor forest (unary chains composed as clamped affine
+     * functions of the stone fraction, in synchronous rounds), 0 (default)
+     * runs the dependency engine as the FP64 form does; cells computed and
+     * the per-cell arithmetic of every other field are the same, stone values
+     * within a few float ulps.  The contraction is faster on one DEM alone
+     * and slower on atlases stepped together (its first rounds are
+     * bandwidth-bound gathers that compete with the other atlases' streams) */
+    int fp32_tree_sediment;
 } lem_tuning;
 int lemhip_get_tuning(lemhip_ctx *ctx, lem_tuning *out);
 int lemhip_set_tuning(lemhip_ctx *ctx, const lem_tuning *t);
@@ -226,7 +235,8 @@ int lemhip_autotune(lemhip_ctx *ctx);
  * erosion.cu:171-181).  Slots: 0 FD, 1 FlowDirs kernel, 2 plateau rounds,
  * 3 depression routing, 4 runoff kernel, 5 flow accumulation, 6 MOD part 1
  * kernel, 7 sediment routing, 8 MOD part 2 kernel, 9 reductions, 10 step,
- * 11 flow-accumulation round kernels only, 12 sediment round kernels only.
+ * 11 flow-accumulation round kernels only, 12 sediment round kernels only,
+ * 13 plateau round kernels only, 14 spill-height relaxation kernels only.
  * enable resets the accumulators; read returns milliseconds and launch
  * counts per slot. */
 int lemhip_profile_enable(lemhip_ctx *ctx, int on);
@@ -256,7 +266,8 @@ int lemhip_mem_info(int device_id, double *free_bytes, double *total_bytes);
  * admission by footprint (no reference counterpart).  The edge and label
  * arrays of depression routing grow with the number of sinks and are not
  * included; *grow_per_cell bounds them per cell for DEMs with at most one
- * sink label per 16 cells. */
+ * sink label per 16 cells.  Turning on lem_tuning.fp32_tree_sediment adds
+ * 64 bytes per cell on first use. */
 int lemhip_bytes_per_cell(int precision, int atlas, double *fixed_per_cell, double *grow_per_cell);
 
 #ifdef __cplusplus

@@ -194,7 +194,7 @@ static void free_all(lemhip_ctx *c)
                     c->hkeys, c->hvals, c->elo, c->ehi, c->ew, c->rmm, c->seen, c->status, c->comp, c->cmax,
                     c->dmask, c->dag_list[0], c->dag_list[1], c->dag_list[2], c->dag_wake,
                     c->dag_ctl, c->partials, c->scal, c->demid, c->ddesc, c->dsc, c->slot_flags, c->labdem,
-                    c->dstage, c->tcache, c->fx_shift, c->fx_tag_cell, c->fx_tag_sum};
+                    c->dstage, c->tcache, c->fx_shift, c->fx_tag_cell, c->fx_tag_sum, c->cs_rec};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int q = 0; q < 2; q++) {
@@ -298,7 +298,7 @@ static int alloc_ctx(lemhip_ctx *c, const lem_params *p, int W, int H, int preci
     for (int q = 0; q < 3; q++) LEM_TRY(A((void **)&c->dag_list[q], ntiles * 4));
     LEM_TRY(A((void **)&c->dag_wake, ntiles * 4));
     LEM_CHECK(c, hipMemset(c->dag_wake, 0, ntiles * 4));
-    LEM_TRY(A(&c->dag_ctl, 256));
+    LEM_TRY(A(&c->dag_ctl, 512));
     LEM_TRY(A((void **)&c->partials, 3L * SLOT_KB * (LEM_MAX_SLOTS + 1) * 8));
     LEM_TRY(A((void **)&c->fx_shift, sizeof(int) * (LEM_MAX_SLOTS + 1)));
     // tagged cells of the integer flow accumulation: one path per DEM (longer
@@ -878,6 +878,7 @@ extern "C" int lemhip_get_tuning(lemhip_ctx *c, lem_tuning *o)
     for (int k = 0; k < 16; k++) o->tuned_ms[k] = c->tune_ms[k];
     o->fp32_integer_fa = c->fx_enabled ? 1 : 0;
     o->use_graphs = c->use_graphs ? 1 : 0;
+    o->fp32_tree_sediment = c->cs_enabled ? 1 : 0;
     return LEMHIP_OK;
 }
 
@@ -896,6 +897,10 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     if (k != 4 && k != 8 && k != 16) { c->err = "plateau rounds per launch must be 4, 8 or 16"; return LEMHIP_E_ARG; }
     if (t->fp32_integer_fa != 0 && t->fp32_integer_fa != 1) { c->err = "fp32_integer_fa must be 0 or 1"; return LEMHIP_E_ARG; }
     if (t->use_graphs != 0 && t->use_graphs != 1) { c->err = "use_graphs must be 0 or 1"; return LEMHIP_E_ARG; }
+    if (t->fp32_tree_sediment != 0 && t->fp32_tree_sediment != 1) {
+        c->err = "fp32_tree_sediment must be 0 or 1";
+        return LEMHIP_E_ARG;
+    }
     c->use_graphs = t->use_graphs != 0;
     // an explicit choice replaces a pending autotune
     c->tune_dag_pending = false;
@@ -904,6 +909,7 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     c->dag_batch = t->dag_batch;
     c->plat_k = k;
     c->fx_enabled = t->fp32_integer_fa != 0;
+    c->cs_enabled = t->fp32_tree_sediment != 0;
     return LEMHIP_OK;
 }
 

@@ -10,7 +10,7 @@
 // live timing of selected kernels/phases with HIP events on the context stream
 enum {
     PROF_FD = 0, PROF_FLOWDIRS, PROF_PLATEAU, PROF_FLOOD, PROF_RUNOFF, PROF_FA, PROF_MOD1, PROF_SED,
-    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_FA_K, PROF_SED_K, PROF_COUNT
+    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_FA_K, PROF_SED_K, PROF_PLAT_K, PROF_MM_K, PROF_COUNT
 };
 #define PROF_MAXP 256
 struct ProfSlot {
@@ -151,6 +151,9 @@ struct lemhip_ctx {
     unsigned long long *fx_tag_sum = nullptr;
     long fx_tag_cap = 0;
     bool fx_enabled = true;           // false: the dependency engine in every form
+    // FP32 sediment routing as a contraction of the donor graph (lem_sed32.hip)
+    void *cs_rec = nullptr;           // 2 x N unary records, allocated on first use
+    bool cs_enabled = false;          // true: the contraction in the FP32 form under SFD
     // MOD's straight-line segments (part 1 with its totals; the totals,
     // part 2 and the outlet bookkeeping) replayed from HIP graphs
     bool use_graphs = false;
@@ -222,6 +225,7 @@ int lem_fd_phase(lemhip_ctx *c, int iter);
 int lem_runoff_phase(lemhip_ctx *c);
 int lem_calcwater(lemhip_ctx *c);
 int lem_fa32_sfd(lemhip_ctx *c, int *fallback);
+int lem_sed32(lemhip_ctx *c, int *fallback);
 int lem_fa_phase(lemhip_ctx *c, int iter);
 int lem_mod_phase(lemhip_ctx *c, int iter);
 int lem_sync_scalars(lemhip_ctx *c);

@@ -13,7 +13,6 @@
 #include "lem_ctx.h"
 #include <string.h>
 
-#define DAG_PENDING 0x7FFFFFFF
 #define DAG_TILE_DONE 0x7FFFFFFF   // wake stamp of a tile with no pending cell
 // status word: DAG_PENDING, r >= 0 done in round r (0 = before the first
 // round), -(r+1) dead (can never complete) since round r
@@ -341,7 +340,9 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
                                                   const DemDesc *__restrict__ dd, uint8_t *__restrict__ dm,
                                                   int *__restrict__ st, const T *__restrict__ ero,
                                                   const T *__restrict__ inci, const T *__restrict__ geli, double ddk,
-                                                  double dck, double dgk, T *__restrict__ dps)
+                                                  double dck, double dgk, T *__restrict__ dps,
+                                                  const double *__restrict__ soilT, const T *__restrict__ stone,
+                                                  T *__restrict__ ca, T *__restrict__ cb, int *bad)
 {
     __shared__ uint8_t smk[PREP_RH * PREP_RW], sfd[PREP_RH * PREP_RW];
     __shared__ T se[PREP_RH * PREP_RW], si[PREP_RH * PREP_RW], sg[PREP_RH * PREP_RW];
@@ -417,7 +418,26 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
                 accum_c += one * (si[pn] * kc);
                 accum_g += one * (sg[pn] * kg);
             }
-            dps[i] = ((accum_d + se[p]) * (T)ddk) + ((accum_c + si[p]) * (T)dck) + ((accum_g + sg[p]) * (T)dgk);
+            const T dp = ((accum_d + se[p]) * (T)ddk) + ((accum_c + si[p]) * (T)dck) + ((accum_g + sg[p]) * (T)dgk);
+            dps[i] = dp;
+            // the stone-mixing weights of a cell that can compute, as the
+            // engine's cell body forms them (the contraction of the FP32
+            // form: a pending cell that mixes is marked DAG_PENDING - 1)
+            if (ca && !dead) {
+                const double sT = soilT[i];
+                const T sv = stone[i];
+                const double sedthick = sT + (double)dp;
+                T a = sv, b = 0;
+                if (sedthick != 0.0) {
+                    const double old = sT / sedthick;
+                    a = (T)old * sv;
+                    b = (T)(1 - old);
+                    s = DAG_PENDING - 1;
+                }
+                ca[i] = a;
+                cb[i] = b;
+                if (!__builtin_isfinite(a) || !__builtin_isfinite(b)) *bad = 1;
+            }
         }
         dm[i] = (uint8_t)bits;
         st[i] = s;
@@ -863,8 +883,10 @@ int lem_fa_dag(lemhip_ctx *c)
     }
 }
 
+// donor bits (dmask), states (scan_tmp) and deposition (z2 as T) of the
+// sediment pass
 template <typename T>
-int lem_sed_dag(lemhip_ctx *c)
+int lem_sed_prep(lemhip_ctx *c, T *ca, T *cb, int *bad)
 {
     dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 4 * PREP_RG - 1) / (4 * PREP_RG));
     int *st = reinterpret_cast<int *>(c->scan_tmp);
@@ -875,8 +897,15 @@ int lem_sed_dag(lemhip_ctx *c)
     // free until the next step
     k_sed_prep<T><<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->demid, c->ddesc, c->dmask, st,
                                          (const T *)c->ero, (const T *)c->inci, (const T *)c->geli, ddk, dck, dgk,
-                                         (T *)c->z2);
+                                         (T *)c->z2, c->soilT, (const T *)c->stone, ca, cb, bad);
     LEM_LAUNCHED(c);
+    return LEMHIP_OK;
+}
+
+template <typename T>
+int lem_sed_dag(lemhip_ctx *c)
+{
+    LEM_TRY(lem_sed_prep<T>(c, nullptr, nullptr, nullptr));
     switch (c->dag_shape) {
 #define X(id, tw, th) case id: return sed_shape<T, tw, th>(c);
     DAG_SHAPES(X)
@@ -944,3 +973,4 @@ template int lem_fa_dag<double>(lemhip_ctx *);
 template int lem_fa_dag<float>(lemhip_ctx *);
 template int lem_sed_dag<double>(lemhip_ctx *);
 template int lem_sed_dag<float>(lemhip_ctx *);
+template int lem_sed_prep<float>(lemhip_ctx *, float *, float *, int *);

@@ -47,6 +47,9 @@ struct DemDesc {
     int x0, y0, w, h;
 };
 #define DEM_NONE 255
+// sediment prep states: pending (DAG_PENDING, or DAG_PENDING - 1 for a cell
+// whose stone mixes, where the FP32 contraction asks for it), -1 dead
+#define DAG_PENDING 0x7FFFFFFF
 #define LEM_MAX_SLOTS 255
 
 // per-slot scalars (the reference's per-DEM host bookkeeping)

@@ -1123,6 +1123,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     int r = 0;  // rounds done in this pass
     long guard = 0;
     for (;;) {
+        prof_begin(c, PROF_PLAT_K);
         for (int q = 0; q < BATCH; q++) {
             const int *a_cur = L[launch % 3];
             int *a_next = L[(launch + 1) % 3];
@@ -1142,6 +1143,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
             ++launch;
             r += c->plat_k;
         }
+        prof_end(c, PROF_PLAT_K, BATCH);
         LEM_LAUNCHED(c);
         LEM_CHECK(c, hipMemcpyAsync(&c->hint[24], ctl, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s));
         LEM_CHECK(c, hipMemcpyAsync(&c->hint[27], &ctl->last_change, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -1378,10 +1380,13 @@ static int flooding(lemhip_ctx *c, int ncand1)
         LEM_CHECK(c, hipMemcpyAsync(reached, &c->hint[40], sizeof(uint32_t), hipMemcpyHostToDevice, c->s));
         for (int it = 0;; it += 16) {
             LEM_TRY(clear_flag(c));
-            if (na > 0)
+            if (na > 0) {
+                prof_begin(c, PROF_MM_K);
                 for (int q = 0; q < 16; q++)
                     k_minimax_relax_reached<<<grid1d(na, 256, 1 << 30), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, reached,
                                                                                          &c->scal->flag, q & 1);
+                prof_end(c, PROF_MM_K, 16);
+            }
             LEM_LAUNCHED(c);
             uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
             unsigned long long *w2 = c->ew2[buf];

@@ -703,7 +703,11 @@ static int mod_t(lemhip_ctx *c)
     }));
     if (c->use_graphs) prof_end(c, PROF_MOD1);
     prof_begin(c, PROF_SED);
-    LEM_TRY(lem_sed_dag<T>(c));
+    {
+        int fallback = 1;
+        if (sizeof(T) == 4 && c->p.flowdirectiontype == 0 && c->cs_enabled) LEM_TRY(lem_sed32(c, &fallback));
+        if (fallback) LEM_TRY(lem_sed_dag<T>(c));
+    }
     prof_end(c, PROF_SED, c->dag_rounds_last);
     // the low-soil variant sits one allocated grid after the other, whatever
     // band of rows this step covers (an atlas steps only its used rows, and

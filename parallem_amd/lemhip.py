@@ -47,19 +47,19 @@ EXPORTS = [
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
-              "mod2", "reductions", "step", "fa_rounds", "sed_rounds"]
+              "mod2", "reductions", "step", "fa_rounds", "sed_rounds", "plat_rounds", "spill_relax"]
 
 
 class LemTuning(C.Structure):
     _fields_ = [("dag_tile_w", C.c_int), ("dag_tile_h", C.c_int), ("dag_batch", C.c_int),
                 ("plateau_rounds_per_launch", C.c_int), ("tuned_ms", C.c_double * 16),
-                ("fp32_integer_fa", C.c_int), ("use_graphs", C.c_int)]
+                ("fp32_integer_fa", C.c_int), ("use_graphs", C.c_int), ("fp32_tree_sediment", C.c_int)]
 
     def as_dict(self) -> dict:
         return {"dag_tile": [self.dag_tile_w, self.dag_tile_h], "dag_batch": self.dag_batch,
                 "plateau_rounds_per_launch": self.plateau_rounds_per_launch,
                 "tuned_ms": [round(x, 3) for x in self.tuned_ms], "fp32_integer_fa": self.fp32_integer_fa,
-                "use_graphs": self.use_graphs}
+                "use_graphs": self.use_graphs, "fp32_tree_sediment": self.fp32_tree_sediment}
 
 
 class LemError(RuntimeError):
@@ -230,9 +230,11 @@ class _Handle:
         return t.as_dict()
 
     def set_tuning(self, dag_tile=None, dag_batch=None, plateau_rounds_per_launch=None, fp32_integer_fa=None,
-                   use_graphs=None, **_ignored) -> None:
+                   use_graphs=None, fp32_tree_sediment=None, **_ignored) -> None:
         t = LemTuning()
         self._check(self.L.lemhip_get_tuning(self.h, C.byref(t)), "get_tuning")
+        if fp32_tree_sediment is not None:
+            t.fp32_tree_sediment = int(fp32_tree_sediment)
         if use_graphs is not None:
             t.use_graphs = int(use_graphs)
         if fp32_integer_fa is not None:

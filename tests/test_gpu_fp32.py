@@ -241,6 +241,47 @@ def test_integer_accumulation_equals_the_engine():
     assert ctx[1].tuning()["fp32_integer_fa"] == 1 and ctx[0].tuning()["fp32_integer_fa"] == 0
 
 
+def test_tree_sediment_equals_the_engine():
+    """FP32 form, single flow directions: sediment routing by contraction of
+    the donor forest against the dependency engine, one erosion phase at a
+    time from identical state, the outlet cycle included (runiniter = 0).
+    The same cells compute; deposition, soil thickness and every cell whose
+    stone value is reached without a composed chain are bit-identical; stone
+    values differ by at most 1e-6 relative (a few float ulps)."""
+    dem = synth.fbm_512(side=640, seed_offset=63)
+    H, W = dem.z.shape
+    ctx = {}
+    for tree in (1, 0):
+        c = lemhip.LemContext(lemhip.default_params(W, H, 0, runiniter=0), lemhip.FP32_KAHAN, 0)
+        c.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+        c.set_tuning(fp32_tree_sediment=tree)
+        ctx[tree] = c
+    m = ctx[0].download("mask") == 1
+    for k in range(3):
+        if k:
+            for f in STATE:
+                ctx[1].upload(f, ctx[0].download(f))
+            ctx[1].outlet = ctx[0].outlet
+        for c in ctx.values():
+            c.flow_direction(-1000 + k)
+            c.runoff()
+            c.flow_accum(-1000 + k)
+            c.erosion(-1000 + k)
+        t, e = ctx[1], ctx[0]
+        assert t.summary()["sed_uncomputed"] == e.summary()["sed_uncomputed"]
+        for f in ("depo", "soilT", "ero", "inci", "geli"):
+            assert np.array_equal(t.download(f), e.download(f), equal_nan=True), f"step {k}: {f}"
+        a, b = t.download("stone")[m], e.download("stone")[m]
+        rel = float(np.max(np.abs(a.astype(np.float64) - b) / np.abs(b)))
+        same = float(np.mean(a == b))
+        print(f"\n[fp32] tree vs engine sediment, step {k}: stone max rel {rel:.2e}, bit-equal {same:.4f}, "
+              f"uncomputed {t.summary()['sed_uncomputed']}")
+        assert rel <= 1e-6
+        fa_, fb_ = t.download("fines")[m], e.download("fines")[m]
+        assert float(np.max(np.abs(fa_ - fb_))) <= 1e-4
+    assert ctx[1].tuning()["fp32_tree_sediment"] == 1 and ctx[0].tuning()["fp32_tree_sediment"] == 0
+
+
 def test_integer_accumulation_falls_back_on_negative_weights():
     """A negative rain cell gives a negative runoff weight, which the
     integer sums cannot hold: the step falls back to the engine and still
@@ -283,6 +324,7 @@ def test_integer_accumulation_outlet_cycle():
         orc.step(-1000 + k)
         assert orc.scalars()["fa_uncomputed"] >= 2
         assert ctx.summary()["fa_uncomputed"] == orc.scalars()["fa_uncomputed"]
+        assert ctx.summary()["sed_uncomputed"] == orc.scalars()["sed_uncomputed"]
         a, b = orc["fa"], ctx.download("fa")
         m = orc["mask"] == 1
         assert float(np.max(np.abs(b[m] - a[m]) / a[m])) <= 1e-6

@@ -48,7 +48,7 @@ def _assert_fields(ctx, orc, where, fields=FIELDS_F64, ints=("fd", "SFD", "mask"
 def _assert_summary(ctx, orc, where):
     s = ctx.summary()
     o = orc.scalars()
-    for k in ("sinkcount", "fa_uncomputed", "sed_uncomputed"):
+    for k in ("sinkcount", "fa_uncomputed", "sed_uncomputed", "plateau_rounds1", "plateau_rounds2"):
         assert s[k] == o[k], f"{where}: {k} {s[k]} vs {o[k]}"
     assert s["FA_max"] == o["FA_max"], f"{where}: FA_max"
     for k, ok in (("totwater", "totwater"), ("totE", "totE"), ("totI", "totI"), ("totG", "totG"), ("totD", "totD"),
