@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0
 SIDE = 4096
 
 # Algorithmic bytes per cell-timestep of the whole step (each persistent field
-# read and written once, each hand-off written once and read once), with this
-# build's layouts (u8 fd instead of the reference's i32):
+# read and written once, each hand-off written once and read once), with the
+# layouts used here (u8 fd instead of the reference's i32):
 #   FP64 form: dem 8/8, Slopes 8x8/8x8, soilT 8/8, 6 soil/veg fields 48/48,
 #              4 climate fields 32/0, fd 1/1, fa 8/8                   -> 306 B
 #   FP32 form: dem 8/8, Slopes 8x4/8x4, soilT 8/8, 6 fields 24/24,
@@ -297,7 +297,7 @@ def main():
                     help="tw,th,batch,k: fixed launch tuning instead of the autotuner (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
+        print(json.dumps({"error": "runs on exactly one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

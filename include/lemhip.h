@@ -197,9 +197,10 @@ int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const in
  * idempotent), keep the fastest, and record the times in tuned_ms:
  * [0..5] flow accumulation per tile shape, [8..10] plateau routing per rounds
  * per launch, [11..13] flow accumulation with batches 4, 8, 32.  In the FP32
- * form under single flow directions, flow accumulation does not use the
- * engine by default (fp32_integer_fa below); the tile shape then serves
- * sediment routing. */
+ * form under single flow directions with fp32_integer_fa (below), flow
+ * accumulation does not use the engine; [0..5] and [11..13] then time
+ * sediment routing, the pass the tile shape serves (its in-place fields are
+ * saved and restored around the timed runs). */
 typedef struct {
     int dag_tile_w, dag_tile_h;
     int dag_batch;

@@ -135,7 +135,7 @@ struct lemhip_ctx {
     int dag_rounds_last = 0;
     int plat_k = 8;             // plateau rounds per launch (4, 8 or 16)
     // lemhip_autotune sets both; each tuner clears its own flag once it has run
-    bool tune_dag_pending = false;    // the next flow accumulation times tile shapes and batches
+    bool tune_dag_pending = false;    // the next pass on the engine times tile shapes and batches
     bool tune_plat_pending = false;   // the next plateau pass on a raised surface times rounds per launch
     void *tcache = nullptr;     // MOD part 2's calcite terms per cell (2 x N, T), from the temperatures
     bool tcache_ok = false;     // false after anything that may change a temperature
@@ -235,6 +235,7 @@ int lem_slot_first_lowest(lemhip_ctx *c, const double *z, int s0, int ns);
 int lem_read_slots(lemhip_ctx *c);
 int lem_dag_shape_count();
 int lem_tune_dag(lemhip_ctx *c);
+int lem_tune_sed(lemhip_ctx *c);
 void lem_dag_shape_dims(int id, int *tw, int *th);
 static inline int lem_grid1d(long n, int block = 256, int cap = 8192)
 {

@@ -958,6 +958,72 @@ int lem_tune_dag(lemhip_ctx *c)
     return LEMHIP_OK;
 }
 
+// The same choice where the engine serves only sediment routing (FP32 form
+// under single flow directions, flow accumulation in integers): sediment
+// routing is timed with every tile shape and then with several launch
+// batches.  The pass updates stone, soil thickness, deposition and fines in
+// place, so they are saved first and restored before every timed run and
+// after the last (the step then routes sediment from the state it had).
+int lem_tune_sed(lemhip_ctx *c)
+{
+    const long N = c->N;
+    const size_t fb = (size_t)N * sizeof(float), db = (size_t)N * sizeof(double);
+    char *save = nullptr;
+    LEM_CHECK(c, hipMalloc((void **)&save, 3 * fb + db));
+    struct Field { void *p; size_t bytes; } fields[4] = {{c->stone, fb}, {c->depo, fb}, {c->fines, fb}, {c->soilT, db}};
+    auto copy_all = [&](bool out) -> int {
+        size_t off = 0;
+        for (const Field &f : fields) {
+            void *dst = out ? (void *)(save + off) : f.p;
+            const void *src = out ? f.p : (const void *)(save + off);
+            LEM_CHECK(c, hipMemcpyAsync(dst, src, f.bytes, hipMemcpyDeviceToDevice, c->s));
+            off += f.bytes;
+        }
+        return LEMHIP_OK;
+    };
+    hipEvent_t e0, e1;
+    LEM_CHECK(c, hipEventCreate(&e0));
+    LEM_CHECK(c, hipEventCreate(&e1));
+    auto time_one = [&](float *ms) -> int {
+        LEM_TRY(copy_all(false));
+        LEM_CHECK(c, hipEventRecord(e0, c->s));
+        LEM_TRY(lem_sed_dag<float>(c));
+        LEM_CHECK(c, hipEventRecord(e1, c->s));
+        LEM_CHECK(c, hipEventSynchronize(e1));
+        LEM_CHECK(c, hipEventElapsedTime(ms, e0, e1));
+        return LEMHIP_OK;
+    };
+    LEM_TRY(copy_all(true));
+    float best = 1e30f;
+    int bs = c->dag_shape;
+    c->dag_batch = 16;
+    for (int k = 0; k < lem_dag_shape_count(); k++) {
+        c->dag_shape = k;
+        float ms = 0.f;
+        LEM_TRY(time_one(&ms));
+        c->tune_ms[k] = ms;
+        if (ms < best) { best = ms; bs = k; }
+    }
+    c->dag_shape = bs;
+    static const int BATCHES[3] = {4, 8, 32};
+    int bb = 16;
+    for (int q = 0; q < 3; q++) {
+        c->dag_batch = BATCHES[q];
+        float ms = 0.f;
+        LEM_TRY(time_one(&ms));
+        c->tune_ms[11 + q] = ms;
+        if (ms < best) { best = ms; bb = BATCHES[q]; }
+    }
+    c->dag_batch = bb;
+    LEM_TRY(copy_all(false));
+    LEM_CHECK(c, hipStreamSynchronize(c->s));
+    c->tune_dag_pending = false;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(save);
+    return LEMHIP_OK;
+}
+
 int lem_dag_shape_count() { return 6; }
 void lem_dag_shape_dims(int id, int *tw, int *th)
 {

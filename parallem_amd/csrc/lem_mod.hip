@@ -618,13 +618,16 @@ template <typename T>
 static int fa_t(lemhip_ctx *c)
 {
     using FS = typename FaS<T>::type;
-    if (c->tune_dag_pending) LEM_TRY(lem_tune_dag(c));
+    // the engine's launch choices are timed on the pass that uses it: here
+    // unless flow accumulation runs in integers (then on sediment routing)
+    const bool integer_fa = sizeof(T) == 4 && c->p.flowdirectiontype == 0 && c->fx_enabled;
+    if (c->tune_dag_pending && !integer_fa) LEM_TRY(lem_tune_dag(c));
     prof_begin(c, PROF_FA);
     // FP32 form, single flow directions: exact integer subtree sums
     // (lem_fa32.hip); otherwise, and for inputs those cannot represent, the
     // dependency engine
     int fallback = 1;
-    if (sizeof(T) == 4 && c->p.flowdirectiontype == 0 && c->fx_enabled) LEM_TRY(lem_fa32_sfd(c, &fallback));
+    if (integer_fa) LEM_TRY(lem_fa32_sfd(c, &fallback));
     if (fallback) LEM_TRY(lem_fa_dag<T>(c));
     prof_end(c, PROF_FA, c->dag_rounds_last);
     // FA_max and the first-iteration outlet rule (mfd_simple.cu:332-357)
@@ -702,6 +705,11 @@ static int mod_t(lemhip_ctx *c)
         return slot_sum<T>(c, c->ero, DS_OFF(totE), c->inci, DS_OFF(totI), c->geli, DS_OFF(totG));
     }));
     if (c->use_graphs) prof_end(c, PROF_MOD1);
+    if (c->tune_dag_pending) {
+        // still pending only in the FP32 form with integer flow accumulation
+        if (sizeof(T) == 4 && !(c->p.flowdirectiontype == 0 && c->cs_enabled)) LEM_TRY(lem_tune_sed(c));
+        c->tune_dag_pending = false;
+    }
     prof_begin(c, PROF_SED);
     {
         int fallback = 1;

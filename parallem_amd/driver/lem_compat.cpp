@@ -31,7 +31,7 @@ static int fail(Data *data, const char *what, int rc)
 void createfilenamespace(Data *data)
 {
     // the reference mallocs its file-name buffers here (memory.cu); the
-    // names live inside Data in this build
+    // names live inside Data here
     (void)data;
 }
 

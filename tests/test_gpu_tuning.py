@@ -74,3 +74,26 @@ def test_autotune_runs_once(precision):
     t = ctx.tuning()
     assert tuple(t["dag_tile"]) == other and t["plateau_rounds_per_launch"] == 4
     ctx.close()
+
+
+def test_fp32_sediment_tuning_restores_the_state():
+    """FP32 form with integer flow accumulation: the autotuner times sediment
+    routing on the step's own data, saving and restoring the fields the pass
+    updates in place; the tuned step must equal an untuned one bit for bit."""
+    dem = synth.fbm_512(side=300, seed_offset=35)
+    H, W = dem.z.shape
+    out = []
+    for tune in (True, False):
+        ctx = lemhip.LemContext(lemhip.default_params(W, H, 0), lemhip.FP32_KAHAN, 0)
+        ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+        if tune:
+            ctx.autotune()
+        for k in range(2):
+            ctx.step(-1000 + k)
+        out.append({f: ctx.download(f) for f in FIELDS})
+        if tune:
+            t = ctx.tuning()
+            assert all(x > 0 for x in t["tuned_ms"][:6]) and all(x > 0 for x in t["tuned_ms"][11:14])
+        ctx.close()
+    for f in FIELDS:
+        assert np.array_equal(out[0][f], out[1][f], equal_nan=True), f
