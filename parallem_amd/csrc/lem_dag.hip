@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) k_fa_prep(int W, int H, const uint8_t *__
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * (4 * PREP_RG);
     constexpr int SN = PREP_RH * PREP_RW, SPER = (SN + 255) / 256;
-    uint8_t vm[SPER], vf[SPER];
+    uint32_t vm[SPER], vf[SPER];   // 32-bit: byte registers would be packed, with a wait per load
 #pragma unroll
     for (int j = 0; j < SPER; j++) {
         const int q = tid + j * 256;
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * (4 * PREP_RG);
     // every load of the region in flight before the first LDS store
     constexpr int SN = PREP_RH * PREP_RW, SPER = (SN + 255) / 256;
-    uint8_t vm[SPER], vf[SPER];
+    uint32_t vm[SPER], vf[SPER];   // 32-bit: byte registers would be packed, with a wait per load
     T ve[SPER], vi[SPER], vg[SPER];
 #pragma unroll
     for (int j = 0; j < SPER; j++) {
@@ -518,8 +518,11 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
     const int first = act[blockIdx.x];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(round + 2) % 3] = 0;
     const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH;
+    // the next visit's tile is read during this one
+    int tile_next = first;
     for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
-        const int tile = (q0 == (int)blockIdx.x) ? first : act[q0];
+        const int tile = tile_next;
+        tile_next = q0 + (int)gridDim.x < n ? act[q0 + gridDim.x] : 0;
         const int tx0 = (tile % ntx) * TW, ty0 = (tile / ntx) * TH;
         __syncthreads();
         if (threadIdx.x == 0) { any_pending = 0; nspill[0] = 0; nspill[1] = 0; take = 0; exits = 0; }
@@ -527,7 +530,7 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         //    loads in flight before the first wait); a ring cell counts as
         //    final only if it completed in an earlier launch
         int vs[PER];
-        uint8_t vb[PER], vf[PER];
+        uint32_t vb[PER], vf[PER];   // 32-bit: byte registers would be packed, with a wait per load
         typename Cell::Reg vr[PER];
 #pragma unroll
         for (int q = 0; q < PER; q++) {

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_fx_tile(FxArgs a)
         // directions and mask of the tile and its ring, every load in flight
         // before the first store (off-grid positions: no mask, no code)
         {
-            uint8_t vf[NPPER], vm[NPPER];
+            uint32_t vf[NPPER], vm[NPPER];   // 32-bit: byte registers would be packed, with a wait per load
 #pragma unroll
             for (int q = 0; q < NPPER; q++) {
                 const int k = threadIdx.x + q * 256;

@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
             v[q] = lm_nan();
             if (k < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) v[q] = z[(long)gy * W + gx];
         }
-        uint8_t mk[4];
+        uint32_t mk[4];   // 32-bit: byte registers would be packed, with a wait per load
 #pragma unroll
         for (int rg = 0; rg < 4; rg++) {
             int irow = y0 + rg * 4 + threadIdx.y, icol = x0 + threadIdx.x;
@@ -414,34 +414,51 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
     const int n = ctl->cnt[launch % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(launch + 2) % 3] = 0;
     const int ntx = (W + PTW - 1) / PTW, nty = (H + PTH - 1) / PTH;
+    // the next visit's tile is read during this one (the first together with
+    // the count: the grid never exceeds the list's capacity, and an entry
+    // past the count is never used)
+    int tile_next = act[blockIdx.x];
     for (int q0 = blockIdx.x; q0 < n; q0 += gridDim.x) {
-        const int tile = act[q0];
+        const int tile = tile_next;
+        tile_next = q0 + (int)gridDim.x < n ? act[q0 + gridDim.x] : 0;
         const int tx0 = (tile % ntx) * PTW, ty0 = (tile / ntx) * PTH;
         __syncthreads();
         if (threadIdx.x == 0) last = 0;
+        // load the region from both distance buffers, and in the same wait
+        // which buffer holds each of the 3x3 tiles as of the last launch: the
+        // region's loads do not wait for that choice (a visit of the first
+        // launches spends most of its time waiting on its loads)
+        float v0[PER], v1[PER];
+        uint32_t e[PER];   // 32-bit: byte registers would be packed, with a wait per load
+        int part[PER];   // 3x3 tile of the region cell, -1 off the grid
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int idx = threadIdx.x + q * PLAT_NT;
+            int gy = ty0 - K + idx / RW, gx = tx0 - K + idx % RW;
+            v0[q] = 0.f;
+            v1[q] = 0.f;
+            e[q] = 0;
+            part[q] = -1;
+            if (idx < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) {
+                long i = (long)gy * W + gx;
+                int ny = gy < ty0 ? 0 : (gy >= ty0 + PTH ? 2 : 1), nx = gx < tx0 ? 0 : (gx >= tx0 + PTW ? 2 : 1);
+                part[q] = ny * 3 + nx;
+                v0[q] = sp0[i];
+                v1[q] = sp1[i];
+                e[q] = eqb[i];
+            }
+        }
         if (threadIdx.x < 9) {
             int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
             s_buf[threadIdx.x] = (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) ? plat_buf_as_of(tver[ty * ntx + tx], launch) : 0;
         }
         __syncthreads();
-        // load the region; both LDS buffers hold every cell, so cells that cannot
-        // change (no candidate, or no equal-height neighbour) are never rewritten
+        // both LDS buffers hold every cell, so cells that cannot change (no
+        // candidate, or no equal-height neighbour) are never rewritten
         unsigned own = 0;
         float v[PER];
-        uint8_t e[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            int idx = threadIdx.x + q * PLAT_NT;
-            int gy = ty0 - K + idx / RW, gx = tx0 - K + idx % RW;
-            v[q] = 0.f;
-            e[q] = 0;
-            if (idx < RN && gy >= 0 && gy < H && gx >= 0 && gx < W) {
-                long i = (long)gy * W + gx;
-                int ny = gy < ty0 ? 0 : (gy >= ty0 + PTH ? 2 : 1), nx = gx < tx0 ? 0 : (gx >= tx0 + PTW ? 2 : 1);
-                v[q] = (s_buf[ny * 3 + nx] ? sp1 : sp0)[i];
-                e[q] = eqb[i];
-            }
-        }
+        for (int q = 0; q < PER; q++) v[q] = part[q] < 0 ? 0.f : (s_buf[part[q]] ? v1[q] : v0[q]);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             int idx = threadIdx.x + q * PLAT_NT;
@@ -762,7 +779,7 @@ __global__ void __launch_bounds__(256) k_edges_emit(int W, int H, const uint8_t 
             // whose label never makes an edge
             int oth[8];
             double zn[8];
-            uint8_t dn[8];
+            uint32_t dn[8];   // 32-bit: byte registers would be packed, with a wait per load
 #pragma unroll
             for (int d = 0; d < 8; d++) {
                 const int cx = icol + cdx(d), cy = irow + cdy(d);
