@@ -470,6 +470,21 @@ struct DagCtl {
 
 __device__ __forceinline__ void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
+// Appends a workgroup's woken tiles to the next round's list with one
+// atomic (called by every thread of the workgroup, after a barrier).
+#define DAG_QCAP 512
+__device__ __forceinline__ void dag_flush(int *qbuf, int *qn, int *next, int *count)
+{
+    __shared__ int base;
+    const int cnt = *qn;
+    if (threadIdx.x == 0) base = atomicAdd(count, cnt);
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) next[base + k] = qbuf[k];
+    __syncthreads();
+    if (threadIdx.x == 0) *qn = 0;
+    __syncthreads();
+}
+
 // One round over the woken tiles.  Inside a tile every cell gets a counter of
 // donors still unresolved in the tile; cells whose donors in other tiles are
 // not all final from earlier launches are parked for a later round.  A thread
@@ -511,6 +526,10 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
     __shared__ int take;
     __shared__ int any_pending;
     __shared__ unsigned exits;
+    __shared__ int qbuf[DAG_QCAP];   // tiles woken for the next round, not yet appended
+    __shared__ int qn;
+    bool wg_prog = false;            // thread 0: some visit of this workgroup made progress
+    if (threadIdx.x == 0) qn = 0;
     // the list count and this workgroup's first entry are read together (the
     // grid never exceeds the list's capacity; an entry past the count is
     // never used)
@@ -738,29 +757,27 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         }
         if (pend) any_pending = 1;
         int prog = __syncthreads_or(seeds != 0);
-        // wake each neighbour the chains flow into, once per round; wave 0
-        // appends its picks with one atomic
-        if (threadIdx.x < 64) {
-            int t2 = -1;
-            if (threadIdx.x < 9 && ((exits >> threadIdx.x) & 1)) {
-                int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
-                if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
-                    t2 = ty * ntx + tx;
-                    // queued already, or finished
-                    if (atomicMax(&wake[t2], round + 1) >= round + 1) t2 = -1;
-                }
+        // wake each neighbour the chains flow into, once per round: gathered
+        // in LDS and appended to the next list with one atomic per full
+        // buffer (workgroups adding to one counter serialise)
+        if (threadIdx.x < 9 && ((exits >> threadIdx.x) & 1)) {
+            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+            if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
+                const int t2 = ty * ntx + tx;
+                // not queued already, nor finished
+                if (atomicMax(&wake[t2], round + 1) < round + 1) qbuf[atomicAdd(&qn, 1)] = t2;
             }
-            unsigned long long m = __ballot(t2 >= 0);
-            int base = 0;
-            if (threadIdx.x == 0 && m) base = atomicAdd(&ctl->cnt[(round + 1) % 3], __popcll(m));
-            base = __shfl(base, 0);
-            if (t2 >= 0) next[base + __popcll(m & ((1ULL << threadIdx.x) - 1))] = t2;
         }
         if (threadIdx.x == 0) {
-            if (prog) ctl->flag = 1;
+            if (prog) wg_prog = true;
             if (!any_pending) atomicExch(&wake[tile], DAG_TILE_DONE);
         }
+        __syncthreads();
+        if (qn > DAG_QCAP - 9) dag_flush(qbuf, &qn, next, &ctl->cnt[(round + 1) % 3]);
     }
+    __syncthreads();
+    if (qn > 0) dag_flush(qbuf, &qn, next, &ctl->cnt[(round + 1) % 3]);
+    if (threadIdx.x == 0 && wg_prog) ctl->flag = 1;
 }
 
 __global__ void k_count_pending(long N, const uint8_t *mask, const int *st, int *out)

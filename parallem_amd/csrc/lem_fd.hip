@@ -391,6 +391,21 @@ __global__ void k_plat_tver_init(int ntiles, int *tver)
     if (t < ntiles) tver[t] = -2;   // buffer 0, written before any launch
 }
 
+// Appends a workgroup's queued tiles to the next launch's list with one
+// atomic (called by every thread of the workgroup, after a barrier).
+#define PLAT_QCAP 1024
+__device__ __forceinline__ void plat_flush(int *qbuf, int *qn, int *next, int *count)
+{
+    __shared__ int base;
+    const int cnt = *qn;
+    if (threadIdx.x == 0) base = atomicAdd(count, cnt);
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) next[base + k] = qbuf[k];
+    __syncthreads();
+    if (threadIdx.x == 0) *qn = 0;
+    __syncthreads();
+}
+
 // MARKS: the pass records slope marks (the first pass, on the dem; the pass
 // on the raised surface keeps only directions and distances)
 template <int K, typename TS, bool MARKS>
@@ -411,6 +426,10 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
     __shared__ uint8_t code[NI], marks[NI];
     __shared__ int last;
     __shared__ int s_buf[9];
+    __shared__ int qbuf[PLAT_QCAP];   // tiles queued for the next launch, not yet appended
+    __shared__ int qn;
+    int wg_last = 0;                  // thread 0: the last round with a change in this workgroup's visits
+    if (threadIdx.x == 0) qn = 0;
     const int n = ctl->cnt[launch % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(launch + 2) % 3] = 0;
     const int ntx = (W + PTW - 1) / PTW, nty = (H + PTH - 1) / PTH;
@@ -538,25 +557,24 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
         __syncthreads();
         if (threadIdx.x == 0) {
             tver[tile] = (launch << 1) | (s_buf[4] ? 0 : 1);
-            if (last > 0) atomicMax(&ctl->last_change, r0 + last);
+            if (last > 0 && r0 + last > wg_last) wg_last = r0 + last;
         }
-        // still moving at the end of the launch: this tile and its neighbours go on
-        if (last == K && threadIdx.x < 64) {   // wave 0 appends its picks with one atomic
-            int t2 = -1;
-            if (threadIdx.x < 9) {
-                int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
-                if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
-                    t2 = ty * ntx + tx;
-                    if (!tile_has[t2] || atomicExch(&tstamp[t2], launch + 1) == launch + 1) t2 = -1;
-                }
+        // still moving at the end of the launch: this tile and its neighbours
+        // go on, gathered in LDS and appended to the next list with one
+        // atomic per full buffer (workgroups adding to one counter serialise)
+        if (last == K && threadIdx.x < 9) {
+            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+            if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
+                const int t2 = ty * ntx + tx;
+                if (tile_has[t2] && atomicExch(&tstamp[t2], launch + 1) != launch + 1) qbuf[atomicAdd(&qn, 1)] = t2;
             }
-            unsigned long long m = __ballot(t2 >= 0);
-            int base = 0;
-            if (threadIdx.x == 0 && m) base = atomicAdd(&ctl->cnt[(launch + 1) % 3], __popcll(m));
-            base = __shfl(base, 0);
-            if (t2 >= 0) next[base + __popcll(m & ((1ULL << threadIdx.x) - 1))] = t2;
         }
+        __syncthreads();
+        if (qn > PLAT_QCAP - 9) plat_flush(qbuf, &qn, next, &ctl->cnt[(launch + 1) % 3]);
     }
+    __syncthreads();
+    if (qn > 0) plat_flush(qbuf, &qn, next, &ctl->cnt[(launch + 1) % 3]);
+    if (threadIdx.x == 0 && wg_last > 0) atomicMax(&ctl->last_change, wg_last);
 }
 
 __global__ void k_slots_sinks_init(int ns, DemScal *ds)
