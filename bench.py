@@ -67,8 +67,8 @@ def kernel_bytes(kernel, t):
     plat_rounds (k_plat_block, plateau routing on the raised surface): surface
                8, mask 1, equal-height bits 1 read; fd, SFD 1 each and the
                distance 4 written, once per cell.
-    spill_relax (k_minimax_relax_reached) is counted per basin graph, not per
-               cell: see graph_bytes.
+    spill_relax (the Borůvka phases of the spill heights, k_bv_*) is counted
+               per basin graph, not per cell: see graph_bytes.
     """
     return {
         "fa_rounds": 2 + t + 8,
@@ -97,7 +97,7 @@ def rocprof_name(slot, tuning, t=4):
             "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs",
             # pass 2 (raised surface, no slope marks) holds nearly all the rounds
             "plat_rounds": f"k_plat_block<{tuning['plateau_rounds_per_launch']}, {ts}, false>",
-            "spill_relax": "k_minimax_relax_reached"}[slot]
+            "spill_relax": "k_bv_min_w"}[slot]
 
 
 def dems(count):
@@ -179,7 +179,7 @@ def run_batched(ds, precision, steps, warmup, tuning=None, groups=1):
         n = len(part)
         cols = min(range(1, n + 1), key=lambda c: (c * -(-n // c), abs(c - -(-n // c))))
         rows = -(-n // cols)
-        aw, ah = cols * (SIDE + 1), rows * (SIDE + 1)
+        aw, ah = lemhip.atlas_span([SIDE] * cols), lemhip.atlas_span([SIDE] * rows)
         atlas = lemhip.LemAtlas(lemhip.default_params(aw, ah, flowtype=0), aw, ah, precision, 0)
         sl = [atlas.add(d.z, d.rain, soilT_mode="nonneg") for d in part]
         assert all(s is not None for s in sl)

@@ -200,7 +200,8 @@ int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const in
  * form under single flow directions with fp32_integer_fa (below), flow
  * accumulation does not use the engine; [0..5] and [11..13] then time
  * sediment routing, the pass the tile shape serves (its in-place fields are
- * saved and restored around the timed runs). */
+ * saved and restored around the timed runs), unless fp32_tree_sediment is
+ * on (its first round has a fixed tile shape; nothing is timed). */
 typedef struct {
     int dag_tile_w, dag_tile_h;
     int dag_batch;
@@ -216,14 +217,16 @@ typedef struct {
      * of rows or the slot count changes); 0 (default): launched directly.
      * Results are identical. */
     int use_graphs;
-    /* FP32 form with single flow directions: 1 routes sediment by
-     * contracting the donor forest (unary chains composed as clamped affine
-     * functions of the stone fraction, in synchronous rounds), 0 (default)
-     * runs the dependency engine as the FP64 form does; cells computed and
-     * the per-cell arithmetic of every other field are the same, stone values
-     * within a few float ulps.  The contraction is faster on one DEM alone
-     * and slower on atlases stepped together (its first rounds are
-     * bandwidth-bound gathers that compete with the other atlases' streams) */
+    /* FP32 form with single flow directions: 1 (default) routes sediment
+     * with one round of the dependency engine over every 32 x 32 tile (the
+     * cells whose upstream lies inside their tile), then contracts the rest
+     * of the donor forest (unary chains composed as clamped affine functions
+     * of the stone fraction, in synchronous rounds); 0 runs the engine's
+     * rounds to the end as the FP64 form does.  The same cells compute, every
+     * field but stone is bit-identical between the two, and stone values
+     * differ by a few float ulps where a composed chain reached them.  The
+     * tile shape of the first round is fixed (atlas rectangles are placed at
+     * multiples of 32, so a DEM's results do not depend on its rectangle) */
     int fp32_tree_sediment;
 } lem_tuning;
 int lemhip_get_tuning(lemhip_ctx *ctx, lem_tuning *out);
@@ -267,8 +270,9 @@ int lemhip_mem_info(int device_id, double *free_bytes, double *total_bytes);
  * admission by footprint (no reference counterpart).  The edge and label
  * arrays of depression routing grow with the number of sinks and are not
  * included; *grow_per_cell bounds them per cell for DEMs with at most one
- * sink label per 16 cells.  Turning on lem_tuning.fp32_tree_sediment adds
- * 64 bytes per cell on first use. */
+ * sink label per 16 cells.  The FP32 form's fixed figure includes the 64
+ * bytes per cell of sediment records (lem_tuning.fp32_tree_sediment),
+ * allocated on first use. */
 int lemhip_bytes_per_cell(int precision, int atlas, double *fixed_per_cell, double *grow_per_cell);
 
 #ifdef __cplusplus

@@ -37,15 +37,17 @@ def pack(dems, groups: int, width: int):
     atlases = []
     for g in range(groups):
         # shelf heights follow the packing order; a generous height bound, trimmed by the packer's use
+        # (rectangles start at multiples of lemhip.ATLAS_ALIGN)
+        a = lemhip.ATLAS_ALIGN
         x, shelf_h, h_total = 0, 0, 0
         for k in members[g]:
             h, w = dems[k].z.shape
             if x + w + 1 > width:
-                h_total += shelf_h
+                h_total += -(-shelf_h // a) * a
                 x, shelf_h = 0, 0
-            x += w + 1
+            x += -(-(w + 1) // a) * a
             shelf_h = max(shelf_h, h + 1)
-        h_total += shelf_h
+        h_total += -(-shelf_h // a) * a
         # the library's shelf packer may place DEMs differently from this
         # estimate: one more shelf of the tallest DEM covers the difference
         # (a step covers only the rows its shelves use)

@@ -4,6 +4,7 @@
 #include "lem_ctx.h"
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <new>
 #include <vector>
 #include <type_traits>
@@ -127,6 +128,15 @@ __global__ void k_slots_first_iter(int ns, const int *flags, DemScal *ds)
 
 // ---------------------------------------------------------- atlas packing
 
+// Every DEM starts at a column and a row that are multiples of LEM_ALIGN:
+// the fixed 32 x 32 tiling of the FP32 sediment pass's first round then
+// lies in the same place relative to each DEM as in a single-DEM context.
+// A DEM's rectangle with its nodata column takes its width rounded up to a
+// multiple of LEM_ALIGN (less at the grid's right edge), a shelf its height
+// rounded up.
+#define LEM_ALIGN 32
+static inline int align_up(int v) { return (v + LEM_ALIGN - 1) / LEM_ALIGN * LEM_ALIGN; }
+
 bool ShelfPack::alloc(int w, int h, int *x, int *y)
 {
     int pw = w + 1, ph = h + 1;
@@ -144,14 +154,15 @@ bool ShelfPack::alloc(int w, int h, int *x, int *y)
         S.h = ph;
         S.free.push_back({0, W});
         shelves.push_back(S);
-        top += ph;
+        top = std::min(H, top + align_up(ph));
         best = (int)shelves.size() - 1;
         bestk = 0;
     }
     Shelf &S = shelves[best];
     *x = S.free[bestk].first;
     *y = S.y;
-    S.free[bestk].first += pw;
+    // an interval ends at the grid's edge or at another DEM's (aligned) start
+    S.free[bestk].first += std::min(align_up(pw), S.free[bestk].second - S.free[bestk].first);
     if (S.free[bestk].first == S.free[bestk].second) S.free.erase(S.free.begin() + bestk);
     return true;
 }
@@ -159,7 +170,7 @@ bool ShelfPack::alloc(int w, int h, int *x, int *y)
 void ShelfPack::release(int x, int y, int w, int h)
 {
     (void)h;
-    int pw = w + 1;
+    int pw = std::min(align_up(w + 1), W - x);   // what alloc took
     for (int s = 0; s < (int)shelves.size(); s++) {
         Shelf &S = shelves[s];
         if (S.y != y) continue;
@@ -1280,6 +1291,7 @@ extern "C" int lemhip_bytes_per_cell(int precision, int atlas, double *fixed_per
     b += 2 * t;              // calcite terms
     b += 4 + 8 + 1 + 1;      // scan / status words, raised surface, donor bits, label slots
     if (atlas) b += 1;       // slot id
+    if (precision == LEMHIP_FP32_KAHAN) b += 64;   // sediment records, two 32 B buffers
     if (fixed_per_cell) *fixed_per_cell = b;
     // edge lists: 64 B per slot (hash keys and values, pair and weight list,
     // two compaction buffers), 8 slots per label rounded up to a power of

@@ -151,9 +151,10 @@ struct lemhip_ctx {
     unsigned long long *fx_tag_sum = nullptr;
     long fx_tag_cap = 0;
     bool fx_enabled = true;           // false: the dependency engine in every form
-    // FP32 sediment routing as a contraction of the donor graph (lem_sed32.hip)
+    // FP32 sediment routing: the engine's first round, then a contraction of
+    // the donor graph (lem_sed32.hip)
     void *cs_rec = nullptr;           // 2 x N unary records, allocated on first use
-    bool cs_enabled = false;          // true: the contraction in the FP32 form under SFD
+    bool cs_enabled = true;           // false: the engine's rounds to the end in the FP32 form under SFD
     // MOD's straight-line segments (part 1 with its totals; the totals,
     // part 2 and the outlet bookkeeping) replayed from HIP graphs
     bool use_graphs = false;
@@ -226,6 +227,7 @@ int lem_runoff_phase(lemhip_ctx *c);
 int lem_calcwater(lemhip_ctx *c);
 int lem_fa32_sfd(lemhip_ctx *c, int *fallback);
 int lem_sed32(lemhip_ctx *c, int *fallback);
+int lem_sed_first(lemhip_ctx *c);
 int lem_fa_phase(lemhip_ctx *c, int iter);
 int lem_mod_phase(lemhip_ctx *c, int iter);
 int lem_sync_scalars(lemhip_ctx *c);

@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
                                                   const T *__restrict__ inci, const T *__restrict__ geli, double ddk,
                                                   double dck, double dgk, T *__restrict__ dps,
                                                   const double *__restrict__ soilT, const T *__restrict__ stone,
-                                                  T *__restrict__ ca, T *__restrict__ cb, int *bad)
+                                                  int *bad)
 {
     __shared__ uint8_t smk[PREP_RH * PREP_RW], sfd[PREP_RH * PREP_RW];
     __shared__ T se[PREP_RH * PREP_RW], si[PREP_RH * PREP_RW], sg[PREP_RH * PREP_RW];
@@ -421,9 +421,9 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
             const T dp = ((accum_d + se[p]) * (T)ddk) + ((accum_c + si[p]) * (T)dck) + ((accum_g + sg[p]) * (T)dgk);
             dps[i] = dp;
             // the stone-mixing weights of a cell that can compute, as the
-            // engine's cell body forms them (the contraction of the FP32
-            // form: a pending cell that mixes is marked DAG_PENDING - 1)
-            if (ca && !dead) {
+            // engine's cell body forms them: the contraction of the FP32 form
+            // needs them finite (bad is set otherwise)
+            if (bad && !dead) {
                 const double sT = soilT[i];
                 const T sv = stone[i];
                 const double sedthick = sT + (double)dp;
@@ -432,10 +432,7 @@ __global__ void __launch_bounds__(256) k_sed_prep(int W, int H, const uint8_t *_
                     const double old = sT / sedthick;
                     a = (T)old * sv;
                     b = (T)(1 - old);
-                    s = DAG_PENDING - 1;
                 }
-                ca[i] = a;
-                cb[i] = b;
                 if (!__builtin_isfinite(a) || !__builtin_isfinite(b)) *bad = 1;
             }
         }
@@ -812,8 +809,10 @@ __global__ void k_iota(int n, int *a)
     if (i < n) a[i] = i;
 }
 
+// one_launch: only the first round (every tile once), with no wait for the
+// device; *uncomputed is not set
 template <int TW, int TH, class Cell, bool SFD>
-static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
+static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot, bool one_launch = false)
 {
     int ntiles = ((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
     int *st = reinterpret_cast<int *>(c->scan_tmp);
@@ -836,6 +835,17 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot)
         bpc = 4;
     int G = bpc * c->ncu;
     if (G > ntiles) G = ntiles;
+    if (one_launch) {
+        ++r;
+        prof_begin(c, prof_slot);
+        k_dag_round<TW, TH, Cell, SFD><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, L[r % 3],
+                                                       L[(r + 1) % 3], c->dag_wake, ctl);
+        prof_end(c, prof_slot, 1);
+        LEM_LAUNCHED(c);
+        c->dag_rounds_last = 1;
+        c->dag_round = r;
+        return LEMHIP_OK;
+    }
     long guard = 0;
     for (;;) {
         LEM_CHECK(c, hipMemsetAsync(&ctl->flag, 0, sizeof(int), c->s));
@@ -904,9 +914,10 @@ int lem_fa_dag(lemhip_ctx *c)
 }
 
 // donor bits (dmask), states (scan_tmp) and deposition (z2 as T) of the
-// sediment pass
+// sediment pass; bad (optional) is set when a computable cell's
+// stone-mixing weights are not finite
 template <typename T>
-int lem_sed_prep(lemhip_ctx *c, T *ca, T *cb, int *bad)
+int lem_sed_prep(lemhip_ctx *c, int *bad)
 {
     dim3 blk(64, 4), grd((c->W + 63) / 64, (c->H + 4 * PREP_RG - 1) / (4 * PREP_RG));
     int *st = reinterpret_cast<int *>(c->scan_tmp);
@@ -917,7 +928,7 @@ int lem_sed_prep(lemhip_ctx *c, T *ca, T *cb, int *bad)
     // free until the next step
     k_sed_prep<T><<<grd, blk, 0, c->s>>>(c->W, c->H, c->mask, c->fd, c->demid, c->ddesc, c->dmask, st,
                                          (const T *)c->ero, (const T *)c->inci, (const T *)c->geli, ddk, dck, dgk,
-                                         (T *)c->z2, c->soilT, (const T *)c->stone, ca, cb, bad);
+                                         (T *)c->z2, c->soilT, (const T *)c->stone, bad);
     LEM_LAUNCHED(c);
     return LEMHIP_OK;
 }
@@ -925,13 +936,29 @@ int lem_sed_prep(lemhip_ctx *c, T *ca, T *cb, int *bad)
 template <typename T>
 int lem_sed_dag(lemhip_ctx *c)
 {
-    LEM_TRY(lem_sed_prep<T>(c, nullptr, nullptr, nullptr));
+    LEM_TRY(lem_sed_prep<T>(c, nullptr));
     switch (c->dag_shape) {
 #define X(id, tw, th) case id: return sed_shape<T, tw, th>(c);
     DAG_SHAPES(X)
 #undef X
     default: c->err = "unknown tile shape"; return LEMHIP_E_ARG;
     }
+}
+
+// The first round of the engine's sediment pass in the FP32 form under
+// single flow directions, after lem_sed_prep: every 32 x 32 tile once,
+// completing each cell whose donors all lie upstream inside its tile (on
+// the fBm workloads about 80% of the cells).  Cells with a donor path
+// entering from another tile stay pending for the contraction
+// (lem_sed32.hip).  The tile shape is fixed, so which cells complete here
+// depends on the cell's position modulo 32 only.
+int lem_sed_first(lemhip_ctx *c)
+{
+    SedCell<float, 32, 32> cell{reinterpret_cast<const float *>(c->ero), reinterpret_cast<const float *>(c->inci),
+                                reinterpret_cast<const float *>(c->geli), reinterpret_cast<const float *>(c->z2),
+                                reinterpret_cast<float *>(c->depo), reinterpret_cast<float *>(c->stone),
+                                reinterpret_cast<float *>(c->fines), c->soilT};
+    return run_rounds<32, 32, decltype(cell), true>(c, cell, nullptr, PROF_SED_K, true);
 }
 
 // Times flow accumulation with every tile shape, then the best shape with
@@ -1059,4 +1086,4 @@ template int lem_fa_dag<double>(lemhip_ctx *);
 template int lem_fa_dag<float>(lemhip_ctx *);
 template int lem_sed_dag<double>(lemhip_ctx *);
 template int lem_sed_dag<float>(lemhip_ctx *);
-template int lem_sed_prep<float>(lemhip_ctx *, float *, float *, int *);
+template int lem_sed_prep<float>(lemhip_ctx *, int *);

@@ -925,50 +925,157 @@ __global__ void k_minimax_relax(int ne, const uint32_t *elo, const uint32_t *ehi
     if (cb < rb) { atomicMin(&r[b], cb); *flag = 1; }
 }
 
-// The same pass with a bit per label that is set once the label's value is
-// finite (label 0 from the start): an edge between two labels the relaxation
-// has not reached yet can lower nothing, and the bits (one word per 32
-// labels) stay in the level-2 cache where the values do not.  A bit read
-// before another edge sets it in the same pass only defers that edge to the
-// next pass.  Passes alternate the direction in which the grid walks the
-// label-sorted edge list: workgroups run roughly in order, and a value lowered
-// early in a pass is often seen later in the same pass, so chains of basins
-// whose labels fall are carried as far per pass as chains whose labels rise.
-__global__ void k_minimax_relax_reached(int ne, const uint32_t *elo, const uint32_t *ehi, const unsigned long long *ew,
-                                        unsigned long long *r, uint32_t *reached, int *flag, int rev)
+// ---- spill heights by Borůvka phases
+//
+// The bottleneck distance above (the relaxation's fixed point) equals the
+// heaviest edge on the minimum spanning tree's path from label 0, with edges
+// ordered by encoded weight and then by position in the label-sorted list (a
+// strict total order, so the tree is unique).  Borůvka builds that tree in
+// phases: every component takes its lightest outgoing edge, and the chosen
+// edges join components.  In the tree of components the phases form (each
+// component a child of the one it joins, that edge weighing what the
+// component chose), the heaviest edge between two leaves equals the
+// heaviest edge on the spanning tree's path between them (V. King, "A
+// simpler minimum spanning tree verification algorithm", Algorithmica 18,
+// 1997).  So a label v that joins label 0's component in phase p gets
+//     r(v) = max(-inf, the weights v's components chose in phases 1..p,
+//                the weights label 0's components chose in phases 1..p),
+// both running maxima, one per label and one for label 0.  Edges of weight
+// +inf or above (encoded) lower nothing in the relaxation and are left out;
+// a label that never joins keeps +inf (the cut-off fallback below takes
+// those).  About log2(labels) phases instead of one relaxation pass per
+// link of the longest bottleneck path.
+//
+// Per phase (bw/be: the chosen weight and edge per component, double-buffered
+// by phase parity; nxt: the component each root joins):
+__global__ void k_bv_init(long n, uint32_t *comp, unsigned long long *mv, unsigned long long *bw0,
+                          unsigned long long *bw1, uint32_t *be0, uint32_t *be1)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ne) return;
-    if (rev) k = ne - 1 - k;
-    uint32_t a = elo[k], b = ehi[k];
-    bool ba = (reached[a >> 5] >> (a & 31)) & 1u, bb = (reached[b >> 5] >> (b & 31)) & 1u;
-    if (!ba && !bb) return;
-    unsigned long long w = ew[k];
-    unsigned long long ra = r[a], rb = r[b];
-    unsigned long long ca = w > rb ? w : rb;
-    unsigned long long cb = w > ra ? w : ra;
-    if (ca < ra) {
-        atomicMin(&r[a], ca);
-        if (!ba) atomicOr(&reached[a >> 5], 1u << (a & 31));
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        comp[i] = (uint32_t)i;
+        mv[i] = 0ULL;
+        bw0[i] = bw1[i] = ~0ULL;
+        be0[i] = be1[i] = ~0u;
     }
-    if (cb < rb) {
-        atomicMin(&r[b], cb);
-        if (!bb) atomicOr(&reached[b >> 5], 1u << (b & 31));
-    }
-    wave_flag(ca < ra || cb < rb, flag);
 }
 
-// An edge with weight >= the values of both its labels can never lower
-// either (values only decrease and max(w, r) >= w); the relaxation drops such
-// edges between batches of passes and keeps iterating over the rest.
-__global__ void __launch_bounds__(256) k_edges_keep(int ne, const uint32_t *elo, const uint32_t *ehi,
-                                                    const unsigned long long *ew, const unsigned long long *r,
-                                                    uint32_t *lo2, uint32_t *hi2, unsigned long long *w2, int *count)
+// the lightest outgoing weight of each component
+__global__ void k_bv_min_w(int ne, const uint32_t *__restrict__ lo, const uint32_t *__restrict__ hi,
+                           const unsigned long long *__restrict__ w, const uint32_t *__restrict__ comp,
+                           unsigned long long *bw)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    const unsigned long long wk = w[k];
+    const uint32_t ca = comp[lo[k]], cb = comp[hi[k]];
+    if (ca == cb || wk >= INF) return;
+    if (wk < bw[ca]) atomicMin(&bw[ca], wk);
+    if (wk < bw[cb]) atomicMin(&bw[cb], wk);
+}
+
+// the first edge of that weight
+__global__ void k_bv_min_e(int ne, const uint32_t *__restrict__ lo, const uint32_t *__restrict__ hi,
+                           const unsigned long long *__restrict__ w, const uint32_t *__restrict__ comp,
+                           const unsigned long long *__restrict__ bw, uint32_t *be)
+{
+    const unsigned long long INF = ord_enc(lm_inf());
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    const unsigned long long wk = w[k];
+    const uint32_t ca = comp[lo[k]], cb = comp[hi[k]];
+    if (ca == cb || wk >= INF) return;
+    if (wk == bw[ca]) atomicMin(&be[ca], (uint32_t)k);
+    if (wk == bw[cb]) atomicMin(&be[cb], (uint32_t)k);
+}
+
+// a component with a chosen edge points at the component across it (every
+// other label at itself)
+__global__ void k_bv_hook(long n, const uint32_t *__restrict__ lo, const uint32_t *__restrict__ hi,
+                          const uint32_t *__restrict__ comp, const uint32_t *__restrict__ be, uint32_t *nxt)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint32_t e = be[i];
+        uint32_t t = (uint32_t)i;
+        if (e != ~0u) {
+            const uint32_t a = comp[lo[e]], b = comp[hi[e]];
+            t = a == (uint32_t)i ? b : a;
+        }
+        nxt[i] = t;
+    }
+}
+
+// Two components that chose the same edge point at each other (under a
+// strict order no longer cycle forms); the smaller becomes the root.  Only
+// the smaller writes, and the entry it reads is the larger's, which the
+// larger never writes here.
+__global__ void k_bv_break(long n, const uint32_t *__restrict__ be, uint32_t *nxt)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (be[i] == ~0u) continue;
+        const uint32_t o = nxt[i];
+        if (o > (uint32_t)i && nxt[o] == (uint32_t)i) nxt[i] = (uint32_t)i;
+    }
+}
+
+// pointer jumping to the roots of the joined components
+__global__ void k_bv_jump(long n, uint32_t *nxt, int *flag)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint32_t t = nxt[i];
+        const uint32_t tt = nxt[t];
+        const bool ch = tt != t;
+        if (ch) nxt[i] = tt;
+        wave_flag(ch, flag);
+    }
+}
+
+// label 0's running maximum and its new component: st[0], st[1]
+__global__ void k_bv_root0(const uint32_t *comp, const unsigned long long *bw, const uint32_t *be,
+                           const uint32_t *nxt, unsigned long long *st)
+{
+    const uint32_t c0 = comp[0];
+    if (be[c0] != ~0u && bw[c0] > st[0]) st[0] = bw[c0];
+    st[1] = nxt[c0];
+}
+
+// every label: its running maximum, its new component, and its spill height
+// if that component is label 0's; the other parity's entries are reset for
+// the next phase (each label its own)
+__global__ void k_bv_labels(long n, uint32_t *comp, const unsigned long long *__restrict__ bw,
+                            const uint32_t *__restrict__ be, const uint32_t *__restrict__ nxt,
+                            unsigned long long *mv, unsigned long long *r, const unsigned long long *st,
+                            unsigned long long *bwn, uint32_t *ben)
+{
+    const unsigned long long INF = ord_enc(lm_inf()), NEG = ord_enc(-lm_inf());
+    const unsigned long long m0 = st[0];
+    const uint32_t root0 = (uint32_t)st[1];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint32_t c = comp[i];
+        unsigned long long m = mv[i];
+        if (be[c] != ~0u && bw[c] > m) m = bw[c];
+        mv[i] = m;
+        const uint32_t nc = nxt[c];
+        comp[i] = nc;
+        bwn[i] = ~0ULL;
+        ben[i] = ~0u;
+        if (i != 0 && nc == root0 && r[i] == INF) {
+            unsigned long long v = m > m0 ? m : m0;
+            r[i] = v > NEG ? v : NEG;
+        }
+    }
+}
+
+// edges between different components, in order
+__global__ void __launch_bounds__(256) k_bv_keep(int ne, const uint32_t *elo, const uint32_t *ehi,
+                                                 const unsigned long long *ew, const uint32_t *comp, uint32_t *lo2,
+                                                 uint32_t *hi2, unsigned long long *w2, int *count)
 {
     __shared__ int s_wave[4];
     __shared__ int s_base;
+    const unsigned long long INF = ord_enc(lm_inf());
     compact_chunks_ordered<COMPACT_ITEMS>(
-        ne, count, s_wave, &s_base, [&](long k) { unsigned long long w = ew[k]; return w < r[elo[k]] || w < r[ehi[k]]; },
+        ne, count, s_wave, &s_base, [&](long k) { return ew[k] < INF && comp[elo[k]] != comp[ehi[k]]; },
         [&](long k, int at) { lo2[at] = elo[k]; hi2[at] = ehi[k]; w2[at] = ew[k]; });
 }
 
@@ -1273,6 +1380,69 @@ static int cutoff_components(lemhip_ctx *c, int nlab, int ne)
     return LEMHIP_OK;
 }
 
+// Spill heights of the labels into c->rmm (initialised: label 0 -inf, the
+// rest +inf) by Borůvka phases over the label-sorted edge list.
+static int spill_heights(lemhip_ctx *c, int nlab, int ne)
+{
+    const long n = (long)nlab + 1;
+    if (ne <= 0) return LEMHIP_OK;
+    // per-label scratch: running maxima, chosen weights (parity 1) and the two
+    // scalars in the hash value table, chosen edges and roots in the hash key
+    // table (both >= 8 * (nlab + 1) entries of 8 bytes), chosen weights
+    // (parity 0) in cmax
+    unsigned long long *mv = c->hvals, *bw[2] = {c->cmax, c->hvals + n}, *st = c->hvals + 2 * n;
+    uint32_t *be[2] = {reinterpret_cast<uint32_t *>(c->hkeys), reinterpret_cast<uint32_t *>(c->hkeys) + n};
+    uint32_t *nxt = reinterpret_cast<uint32_t *>(c->hkeys) + 2 * n;
+    k_bv_init<<<grid1d(n), 256, 0, c->s>>>(n, c->comp, mv, bw[0], bw[1], be[0], be[1]);
+    LEM_LAUNCHED(c);
+    LEM_CHECK(c, hipMemsetAsync(st, 0, 2 * sizeof(unsigned long long), c->s));
+    const uint32_t *lo = c->elo, *hi = c->ehi;
+    const unsigned long long *w = c->ew;
+    int na = ne, buf = 0;
+    prof_begin(c, PROF_MM_K);
+    long launches = 0;
+    for (int ph = 0; na > 0; ph++) {
+        const int p = ph & 1;
+        const int ge = grid1d(na, 256, 1 << 30);
+        k_bv_min_w<<<ge, 256, 0, c->s>>>(na, lo, hi, w, c->comp, bw[p]);
+        k_bv_min_e<<<ge, 256, 0, c->s>>>(na, lo, hi, w, c->comp, bw[p], be[p]);
+        k_bv_hook<<<grid1d(n), 256, 0, c->s>>>(n, lo, hi, c->comp, be[p], nxt);
+        k_bv_break<<<grid1d(n), 256, 0, c->s>>>(n, be[p], nxt);
+        LEM_LAUNCHED(c);
+        launches += 4;
+        for (int it = 0;; it++) {
+            LEM_TRY(clear_flag(c));
+            for (int q = 0; q < 4; q++) k_bv_jump<<<grid1d(n), 256, 0, c->s>>>(n, nxt, &c->scal->flag);
+            LEM_LAUNCHED(c);
+            launches += 4;
+            int f = 0;
+            LEM_TRY(read_flag(c, &f));
+            if (!f) break;
+            if (it > 16) { c->err = "spill-height phases did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+        }
+        k_bv_root0<<<1, 1, 0, c->s>>>(c->comp, bw[p], be[p], nxt, st);
+        k_bv_labels<<<grid1d(n), 256, 0, c->s>>>(n, c->comp, bw[p], be[p], nxt, mv, c->rmm, st, bw[p ^ 1],
+                                                  be[p ^ 1]);
+        uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
+        unsigned long long *w2 = c->ew2[buf];
+        LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
+        k_bv_keep<<<grid_chunks(na), 256, 0, c->s>>>(na, lo, hi, w, c->comp, lo2, hi2, w2, &c->scal->nedges);
+        LEM_LAUNCHED(c);
+        launches += 3;
+        LEM_TRY(read_scalars(c));
+        const int nk = c->hscal->nedges;
+        // every component with an outgoing edge joins another each phase, so
+        // the components with edges at least halve
+        if (nk >= na && ph > 0) { c->err = "spill-height phases made no progress"; return LEMHIP_E_NOT_CONVERGED; }
+        na = nk;
+        lo = lo2; hi = hi2; w = w2;
+        buf ^= 1;
+        if (ph > 64) { c->err = "spill-height phases did not converge"; return LEMHIP_E_NOT_CONVERGED; }
+    }
+    prof_end(c, PROF_MM_K, launches);
+    return LEMHIP_OK;
+}
+
 // floodingDriver (routeflow.cu:202-676) up to the raised surface in c->z2
 static int flooding(lemhip_ctx *c, int ncand1)
 {
@@ -1402,42 +1572,7 @@ static int flooding(lemhip_ctx *c, int ncand1)
     LEM_LAUNCHED(c);
     c->h_nedges = ne;
 
-    {
-        // relaxation passes over a shrinking set of edges that can still lower a value
-        const uint32_t *lo = c->elo, *hi = c->ehi;
-        const unsigned long long *w = c->ew;
-        int na = ne, buf = 0;
-        // reached-label bits: the component array is free until the cut-off fallback
-        uint32_t *reached = c->comp;
-        const long nw = ((long)nlab + 1 + 31) / 32;
-        LEM_CHECK(c, hipMemsetAsync(reached, 0, nw * sizeof(uint32_t), c->s));
-        c->hint[40] = 1;   // label 0
-        LEM_CHECK(c, hipMemcpyAsync(reached, &c->hint[40], sizeof(uint32_t), hipMemcpyHostToDevice, c->s));
-        for (int it = 0;; it += 16) {
-            LEM_TRY(clear_flag(c));
-            if (na > 0) {
-                prof_begin(c, PROF_MM_K);
-                for (int q = 0; q < 16; q++)
-                    k_minimax_relax_reached<<<grid1d(na, 256, 1 << 30), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, reached,
-                                                                                         &c->scal->flag, q & 1);
-                prof_end(c, PROF_MM_K, 16);
-            }
-            LEM_LAUNCHED(c);
-            uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
-            unsigned long long *w2 = c->ew2[buf];
-            LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
-            if (na > 0)
-                k_edges_keep<<<grid_chunks(na), 256, 0, c->s>>>(na, lo, hi, w, c->rmm, lo2, hi2, w2,
-                                                                      &c->scal->nedges);
-            LEM_LAUNCHED(c);
-            LEM_TRY(read_scalars(c));
-            if (!c->hscal->flag) break;
-            na = c->hscal->nedges;
-            lo = lo2; hi = hi2; w = w2;
-            buf ^= 1;
-            if (it / 16 > nlab + 2) { c->err = "spill heights did not converge"; return LEMHIP_E_NOT_CONVERGED; }
-        }
-    }
+    LEM_TRY(spill_heights(c, nlab, ne));
     LEM_TRY(clear_flag(c));
     if (ne > 0) k_fb_unreached<<<grid1d(ne, 256, 1 << 30), 256, 0, c->s>>>(ne, c->elo, c->ehi, c->rmm, &c->scal->flag);
     LEM_LAUNCHED(c);

@@ -330,6 +330,30 @@ __device__ __forceinline__ void slot_rows(const DemDesc &D, int W, F f)
         for (int c = D.x0 + threadIdx.x; c < D.x0 + D.w; c += blockDim.x) f((long)r * W + c);
 }
 
+// The same cells in the same order per thread, with the loads of U column
+// steps in flight before the first of them is used (one wait per U steps
+// instead of one per step)
+template <int U, typename Ld, typename F>
+__device__ __forceinline__ void slot_rows_u(const DemDesc &D, int W, Ld load, F f)
+{
+    using V = decltype(load(0L));
+    const int kb = slot_kb(D.h), c1 = D.x0 + D.w;
+    for (int r = D.y0 + blockIdx.x; r < D.y0 + D.h; r += kb) {
+        const long row = (long)r * W;
+        for (int c0 = D.x0 + threadIdx.x; c0 < c1; c0 += U * blockDim.x) {
+            V v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int c = c0 + u * blockDim.x;
+                v[u] = c < c1 ? load(row + c) : V{};
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (c0 + u * (int)blockDim.x < c1) f(v[u]);
+        }
+    }
+}
+
 __device__ __forceinline__ bool slot_block_idle(const DemDesc &D) { return D.w == 0 || (int)blockIdx.x >= slot_kb(D.h); }
 
 __device__ __forceinline__ double block_sum_f64(double v, double *sh)
@@ -357,12 +381,21 @@ __global__ void __launch_bounds__(256) k_slot_sum3(const T *__restrict__ a0, con
     DemDesc D = dd[j];
     if (slot_block_idle(D)) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    slot_rows(D, W, [&](long i) {
-        if (mask && !mask[i]) return;
-        s0 += (double)a0[i];
-        if (a1) s1 += (double)a1[i];
-        if (a2) s2 += (double)a2[i];
-    });
+    struct V3 {
+        uint32_t m;
+        T v0, v1, v2;
+    };
+    slot_rows_u<4>(
+        D, W,
+        [&](long i) {
+            return V3{mask ? (uint32_t)mask[i] : 1u, a0[i], a1 ? a1[i] : (T)0, a2 ? a2[i] : (T)0};
+        },
+        [&](const V3 &v) {
+            if (!v.m) return;
+            s0 += (double)v.v0;
+            if (a1) s1 += (double)v.v1;
+            if (a2) s2 += (double)v.v2;
+        });
     double *P = partial + (long)j * 3 * SLOT_KB + blockIdx.x;
     s0 = block_sum_f64(s0, sh);
     if (threadIdx.x == 0) P[0] = s0;
@@ -429,6 +462,11 @@ __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v
     return sh[0];
 }
 
+struct ZM {
+    double z;
+    uint32_t m;
+};
+
 // lowest mask cell with the reference's start value 1000 and strict '<' (NaN never wins)
 __global__ void __launch_bounds__(256) k_slot_lowest(const double *__restrict__ z, const uint8_t *__restrict__ mask,
                                                      int W, const DemDesc *dd, int s0, DemScal *ds)
@@ -439,13 +477,14 @@ __global__ void __launch_bounds__(256) k_slot_lowest(const double *__restrict__ 
     if (slot_block_idle(D)) return;
     unsigned long long best = ord_enc(1000.0);
     if (D.w > 0)
-        slot_rows(D, W, [&](long i) {
-            double v = z[i];
-            if (mask[i] && v < 1000.0) {
-                unsigned long long e = ord_enc(v);
-                if (e < best) best = e;
-            }
-        });
+        slot_rows_u<4>(
+            D, W, [&](long i) { return ZM{z[i], (uint32_t)mask[i]}; },
+            [&](const ZM &q) {
+                if (q.m && q.z < 1000.0) {
+                    unsigned long long e = ord_enc(q.z);
+                    if (e < best) best = e;
+                }
+            });
     best = block_min_u64(best, sh);
     if (threadIdx.x == 0 && D.w > 0) atomicMin(&ds[s].lowest_enc, best);
 }
@@ -512,10 +551,12 @@ __global__ void __launch_bounds__(256) k_slot_famax(const FS *__restrict__ fa, i
     if (slot_block_idle(D)) return;
     unsigned long long best = ord_enc(0.0);
     if (D.w > 0)
-        slot_rows(D, W, [&](long i) {
-            double v = fa_full(fa[i]);
-            if (v > ord_dec(best)) best = ord_enc(v);
-        });
+        slot_rows_u<4>(
+            D, W, [&](long i) { return fa[i]; },
+            [&](const FS &q) {
+                double v = fa_full(q);
+                if (v > ord_dec(best)) best = ord_enc(v);
+            });
     best = block_max_u64(best, sh);
     if (threadIdx.x == 0 && D.w > 0) atomicMax(&ds[s].famax_enc, best);
 }

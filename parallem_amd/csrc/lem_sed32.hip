@@ -1,8 +1,9 @@
 // lem_sed32.hip — sediment routing of the FP32 form under single flow
-// directions (MFDsedbud, mfd_accum.cu:63-252, loop mfd_accum.cu:345-483) as
-// a contraction of the donor forest in synchronous rounds, instead of the
-// dependency engine's tile rounds.  (Under multiple flow directions most
-// cells have several open donors at once, so few chains form and the rounds
+// directions (MFDsedbud, mfd_accum.cu:63-252, loop mfd_accum.cu:345-483):
+// the dependency engine's first round over every tile, then a contraction of
+// what is left of the donor forest in synchronous rounds, instead of the
+// engine's further tile rounds.  (Under multiple flow directions most cells
+// have several open donors at once, so few chains form and the rounds
 // approach the depth of the graph; those contexts keep the engine.)
 //
 // Of a computed cell's results only the stone fraction travels downstream:
@@ -17,12 +18,17 @@
 // or a nodata neighbour in a blocking direction, mfd_accum.cu:42,390-394),
 // and never on or below a cycle.
 //
-// Functions x -> clamp(A + B x, L, U) are closed under composition, so the
+// The engine's first round visits every 32 x 32 tile once and completes the
+// cells whose whole upstream lies inside their tile (about 80% of the cells
+// on the fBm workloads); the later tile rounds would revisit most tiles
+// several times for the rest, and a river crossing hundreds of tiles needs
+// as many rounds.  The rest contracts instead:
+// functions x -> clamp(A + B x, L, U) are closed under composition, so the
 // graph contracts in rounds.  Each cell's state after round r is a function
 // of the states after round r-1 only (status words carry the round of their
 // last change and the kind before it; unary records are double-buffered by
-// round parity), so results do not depend on scheduling, list order, the
-// tiling or the other DEMs of an atlas.  In round r a cell
+// round parity), so results do not depend on scheduling or list order.  In
+// round r a cell
 //   * whose donors have all computed computes, with the engine's arithmetic
 //     (donor values summed in pull order in float): rake;
 //   * with exactly one donor u still open becomes unary: y = F(y_u), F
@@ -34,10 +40,13 @@
 // A pointer that has passed more cells than its DEM holds went round a
 // cycle: the cell dies (never computes).  Rounds end when no cell is open.
 //
-// Values reached by rake are bit-identical with the engine's.  A composed
-// function is evaluated in double from the float parameters, so values
-// reached through one differ from the engine's link-by-link float rounding
-// by a few float ulps.
+// Values of the first round and of rake are bit-identical with the
+// engine's.  A composed function is evaluated in double from the float
+// parameters, so values reached through one differ from the engine's
+// link-by-link float rounding by a few float ulps.  Which cells the first
+// round completes depends on their position modulo the tile (32 cells), so
+// a DEM gives identical results wherever it sits in an atlas only at
+// positions that are multiples of 32.
 #include <algorithm>
 #include "lem_ctx.h"
 
@@ -146,25 +155,17 @@ __device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count
     __syncthreads();
 }
 
-// status of a donor before round 1, read while the first kernel rewrites
-// the words: the prep kernel's -1 (dead) or pending state, or a word already
-// rewritten (stamp 0 for a dead cell; a cell changed in round 1 was open
-// before it)
-__device__ __forceinline__ bool cs_dead_at0(int sn)
-{
-    return sn == -1 || (uint32_t)sn == cs_word(CS_DEAD, CS_DEAD, 0, 0);
-}
-
-// Status words and round 1 in one pass over the cells (the prep kernel has
-// formed the rake parameters and marked the pending cells that mix):
-// before round 1 no donor has computed, so a cell with a dead donor dies, a
-// cell without donors computes from S = 0, and a cell with exactly one donor
-// becomes unary with no sum folded in.  The cells still open form round 2's
-// list.
-__global__ void __launch_bounds__(256) k_cs_first(int W, long N, const uint8_t *__restrict__ mask, int *st,
-                                                  const uint8_t *__restrict__ dm, float *stone,
-                                                  const float *__restrict__ ca, const float *__restrict__ cb,
-                                                  CsRec *rout, int *list, CsCtl *ctl)
+// Status words after the engine's first round (lem_sed_first): a cell it
+// completed is done and one it killed (or the prep kernel marked dead) is
+// dead, both with stamp 0; a pending cell gets its rake parameters, as the
+// engine's cell body forms them from its unchanged inputs, and is open.
+// The open cells form round 2's list.  Each thread rewrites only its own
+// cells' words.
+__global__ void __launch_bounds__(256) k_cs_from_engine(long N, const uint8_t *__restrict__ mask, int *st,
+                                                        const double *__restrict__ soilT,
+                                                        const float *__restrict__ dps,
+                                                        const float *__restrict__ stone, float *ca, float *cb,
+                                                        int *list, CsCtl *ctl)
 {
     __shared__ CsChunk C;
     const long CH = 256L * CS_ITEMS;
@@ -176,46 +177,25 @@ __global__ void __launch_bounds__(256) k_cs_first(int W, long N, const uint8_t *
             if (i < N) {
                 const uint8_t m = mask[i];
                 const int s = st[i];
-                uint32_t w = cs_word(CS_DONE, CS_DONE, 0, 0);   // outside the mask: never a donor
-                if (m && s == -1) {
+                uint32_t w = cs_word(CS_DONE, CS_DONE, 0, 0);   // done, or outside the mask (never a donor)
+                if (m && s == DAG_PENDING) {
+                    const double sT = soilT[i];
+                    const float sv = stone[i];
+                    const double sedthick = sT + (double)dps[i];
+                    float a = sv, b = 0.f;
+                    uint32_t upd = 0;
+                    if (sedthick != 0.0) {
+                        const double old = sT / sedthick;
+                        a = (float)old * sv;
+                        b = (float)(1 - old);
+                        upd = CS_UPD;
+                    }
+                    ca[i] = a;
+                    cb[i] = b;
+                    w = cs_word(CS_PEND, CS_PEND, upd, 0);
+                    keep = true;
+                } else if (m && s < 0) {
                     w = cs_word(CS_DEAD, CS_DEAD, 0, 0);
-                } else if (m) {
-                    const uint32_t upd = s == DAG_PENDING - 1 ? CS_UPD : 0u;
-                    const float a = ca[i], b = cb[i];
-                    const unsigned bits = dm[i];
-                    int cnt = 0, u = -1;
-                    bool dead = false;
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        if (!((bits >> q) & 1u)) continue;
-                        const int d = cpull(q);
-                        const int nb = (int)i + cdy(d) * W + cdx(d);
-                        if (cs_dead_at0(st[nb])) dead = true;
-                        else { cnt++; u = nb; }
-                    }
-                    uint32_t nk = CS_PEND;
-                    if (dead) {
-                        nk = CS_DEAD;
-                    } else if (cnt == 0) {
-                        float y = a;
-                        if (upd) {
-                            y = a + b * 0.f;
-                            if (y < 0.01f) y = 0.01f;
-                            else if (y > 99.9f) y = 99.9f;
-                        }
-                        stone[i] = y;
-                        nk = CS_DONE;
-                    } else if (cnt == 1) {
-                        CsRec f;
-                        if (upd) { f.A = (double)a; f.B = b; f.L = 0.01f; f.U = 99.9f; }
-                        else { f.A = a; f.B = 0.0; f.L = a; f.U = a; }
-                        f.ptr = u;
-                        f.hop = 1;
-                        rout[i] = f;
-                        nk = CS_UN;
-                    }
-                    keep = nk == CS_PEND || nk == CS_UN;
-                    w = nk == CS_PEND ? cs_word(CS_PEND, CS_PEND, upd, 0) : cs_word(nk, CS_PEND, upd, 1);
                 }
                 st[i] = (int)w;
             }
@@ -317,9 +297,10 @@ __global__ void __launch_bounds__(256) k_cs_round(int W, uint32_t r, const int *
     }
 }
 
-// Results of the computed cells as the engine's cell body writes them
-// (deposition, fines, soil thickness; stone is already in place); every
-// other mask cell is left as it was and counted.
+// Results of the cells computed by the rounds as the engine's cell body
+// writes them (deposition, fines, soil thickness; stone is already in
+// place); the engine's first round has written its own cells (done with
+// stamp 0); every other mask cell is left as it was and counted.
 __global__ void __launch_bounds__(256) k_cs_finish(long N, const uint8_t *__restrict__ mask,
                                                    const uint32_t *__restrict__ stw, const float *__restrict__ dps,
                                                    const float *__restrict__ ero, const float *__restrict__ inci,
@@ -333,6 +314,7 @@ __global__ void __launch_bounds__(256) k_cs_finish(long N, const uint8_t *__rest
         const uint32_t w = stw[i];
         if (!m) continue;
         if ((w & 3u) != CS_DONE) { unc++; continue; }
+        if ((w >> 5) == 0) continue;
         const float dp = dps[i];
         const double sT = soilT[i];
         const double sedthick = sT + (double)dp;
@@ -349,7 +331,7 @@ __global__ void __launch_bounds__(256) k_cs_finish(long N, const uint8_t *__rest
     if ((threadIdx.x & 63) == 0 && unc) atomicAdd(&ctl->unc, unc);
 }
 
-template <typename T> int lem_sed_prep(lemhip_ctx *c, T *ca, T *cb, int *bad);
+template <typename T> int lem_sed_prep(lemhip_ctx *c, int *bad);
 
 int lem_sed32(lemhip_ctx *c, int *fallback)
 {
@@ -374,21 +356,25 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
     static_assert(sizeof(CsCtl) <= 512, "control block");
 
     LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(CsCtl), c->s));
-    LEM_TRY(lem_sed_prep<float>(c, ca, cb, &ctl->bad));
+    LEM_TRY(lem_sed_prep<float>(c, &ctl->bad));
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[25], &ctl->bad, sizeof(int), hipMemcpyDeviceToHost, c->s));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     // non-finite parameters: the engine's path handles them (nothing of the
     // state has changed yet)
     if (c->hint[25]) return LEMHIP_OK;
+    *fallback = 0;
+    // round 1: the engine's first round over every tile (its control block
+    // shares the memory of this one)
+    LEM_TRY(lem_sed_first(c));
     const long CH = 256L * CS_ITEMS;
     auto chunks = [&](long n) { return (int)std::max(1L, std::min((n + CH - 1) / CH, 4096L)); };
-    // round 1 (lists[0], records of parity 1, count for round 2 in cnt[2])
-    k_cs_first<<<chunks(N), 256, 0, c->s>>>(c->W, N, c->mask, c->scan_tmp, c->dmask, (float *)c->stone, ca, cb,
-                                            rec[1], lists[0], ctl);
+    LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(CsCtl), c->s));
+    // status words and round 2's list (lists[0], count in cnt[2])
+    k_cs_from_engine<<<chunks(N), 256, 0, c->s>>>(N, c->mask, c->scan_tmp, c->soilT, dps, (const float *)c->stone,
+                                                  ca, cb, lists[0], ctl);
     LEM_LAUNCHED(c);
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[24], &ctl->cnt[2], sizeof(int), hipMemcpyDeviceToHost, c->s));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
-    *fallback = 0;
     long open = c->hint[24];
     uint32_t r = 1;
     long launches = 1;

@@ -342,10 +342,26 @@ class LemContext(_Handle):
         self._restore(0, snap)
 
 
+ATLAS_ALIGN = 32   # DEM rectangles start at columns and rows that are multiples of this
+
+
+def atlas_span(sides) -> int:
+    """Grid columns (or rows) that DEMs of the given widths (heights) take
+    side by side in an atlas: each but the last is padded with its nodata
+    column and rounded up to a multiple of ATLAS_ALIGN, the last needs its
+    nodata column only."""
+    sides = list(sides)
+    if not sides:
+        return 0
+    a = ATLAS_ALIGN
+    return sum(-(-(s + 1) // a) * a for s in sides[:-1]) + sides[-1] + 1
+
+
 class LemAtlas(_Handle):
     """Many DEMs of different sizes, each in its own rectangle ("slot") of one
     device grid, stepped together.  Per-slot results equal those of a
-    LemContext on the same inputs."""
+    LemContext on the same inputs.  Rectangles are shelf-packed at columns
+    and rows that are multiples of ATLAS_ALIGN (see atlas_span)."""
 
     MAX_SLOTS = 255
 

@@ -68,7 +68,7 @@ def test_alone_equals_in_mix(precision, flowtype):
         # runiniter 0: iteration -1000 is each DEM's first, so the outlet rule fires per DEM
         return lemhip.default_params(W, H, flowtype, runiniter=0)
 
-    atlas = lemhip.LemAtlas(params(400, 400), 400, 400, precision, 0)
+    atlas = lemhip.LemAtlas(params(400, 480), 400, 480, precision, 0)
     alone, slots = {}, {}
     for k in ("a", "b", "c"):
         slots[k] = atlas.add(dems[k].z, dems[k].rain, soilT_mode="nonneg")
@@ -101,14 +101,14 @@ def test_alone_equals_in_mix(precision, flowtype):
 
 
 def test_atlas_room_and_state_errors():
-    p = lemhip.default_params(64, 64, 0)
-    atlas = lemhip.LemAtlas(p, 64, 64, lemhip.FP64, 0)
+    p = lemhip.default_params(64, 80, 0)
+    atlas = lemhip.LemAtlas(p, 64, 80, lemhip.FP64, 0)
     assert atlas.open(64, 10) is None          # needs one column of padding
     s = atlas.open(63, 40)
     assert s is not None
-    assert atlas.open(63, 30) is None          # 41 + 31 rows > 64
-    t = atlas.open(30, 20)
-    assert t is not None
+    assert atlas.open(63, 30) is None          # shelves start at multiples of 32: 64 + 31 rows > 80
+    t = atlas.open(30, 10)
+    assert t is not None and atlas.rect(t)[:2] == (0, 64)
     atlas.remove(s)
     assert atlas.open(63, 40) is not None      # freed rows are reused
     summ = lemhip.LemSummary()
@@ -132,7 +132,7 @@ def test_closing_top_shelf_keeps_other_slots(precision):
     atlas = lemhip.LemAtlas(params(200, 300), 200, 300, precision, 0)
     sa = atlas.add(low.z, low.rain, soilT_mode="nonneg")
     sb = atlas.add(top.z, top.rain, soilT_mode="nonneg")
-    assert atlas.rect(sa)[1] == 0 and atlas.rect(sb)[1] == 71   # separate shelves, sb on top
+    assert atlas.rect(sa)[1] == 0 and atlas.rect(sb)[1] == 96   # separate shelves (rows aligned to 32), sb on top
     valid = low.z != -9999.0
     assert np.mean(low.z[valid] < 428.0) > 0.2                   # thin-soil cells (formula soil <= 0.1 m)
     alone = Alone(low, params, precision)

@@ -242,8 +242,9 @@ def test_integer_accumulation_equals_the_engine():
 
 
 def test_tree_sediment_equals_the_engine():
-    """FP32 form, single flow directions: sediment routing by contraction of
-    the donor forest against the dependency engine, one erosion phase at a
+    """FP32 form, single flow directions: sediment routing by the engine's
+    first round plus a contraction of the rest of the donor forest (the
+    default) against the engine's rounds to the end, one erosion phase at a
     time from identical state, the outlet cycle included (runiniter = 0).
     The same cells compute; deposition, soil thickness and every cell whose
     stone value is reached without a composed chain are bit-identical; stone

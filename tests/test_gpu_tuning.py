@@ -56,11 +56,15 @@ def test_bad_tuning_is_rejected():
 @pytest.mark.parametrize("precision", [lemhip.FP64, lemhip.FP32_KAHAN])
 def test_autotune_runs_once(precision):
     """lemhip_autotune tunes on the next step only: a shape set afterwards is
-    kept by the following steps and the phase-level calls."""
+    kept by the following steps and the phase-level calls.  (In the FP32
+    form the tile shape serves the engine's sediment rounds, so those are
+    switched on.)"""
     dem = synth.fbm_512(side=200, seed_offset=33)
     H, W = dem.z.shape
     ctx = lemhip.LemContext(lemhip.default_params(W, H, 0), precision, 0)
     ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+    if precision == lemhip.FP32_KAHAN:
+        ctx.set_tuning(fp32_tree_sediment=0)
     ctx.autotune()
     ctx.step(-1000)
     first = ctx.tuning()
@@ -76,16 +80,21 @@ def test_autotune_runs_once(precision):
     ctx.close()
 
 
-def test_fp32_sediment_tuning_restores_the_state():
-    """FP32 form with integer flow accumulation: the autotuner times sediment
-    routing on the step's own data, saving and restoring the fields the pass
-    updates in place; the tuned step must equal an untuned one bit for bit."""
+@pytest.mark.parametrize("tree", [0, 1])
+def test_fp32_sediment_tuning_restores_the_state(tree):
+    """FP32 form with integer flow accumulation: with the engine's sediment
+    rounds (tree 0) the autotuner times sediment routing on the step's own
+    data, saving and restoring the fields the pass updates in place; with
+    the first-round-plus-contraction routing (tree 1, fixed tile shape) it
+    times nothing there.  The tuned step must equal an untuned one bit for
+    bit."""
     dem = synth.fbm_512(side=300, seed_offset=35)
     H, W = dem.z.shape
     out = []
     for tune in (True, False):
         ctx = lemhip.LemContext(lemhip.default_params(W, H, 0), lemhip.FP32_KAHAN, 0)
         ctx.init_state(dem.z, dem.rain, soilT_mode="nonneg")
+        ctx.set_tuning(fp32_tree_sediment=tree)
         if tune:
             ctx.autotune()
         for k in range(2):
@@ -93,7 +102,8 @@ def test_fp32_sediment_tuning_restores_the_state():
         out.append({f: ctx.download(f) for f in FIELDS})
         if tune:
             t = ctx.tuning()
-            assert all(x > 0 for x in t["tuned_ms"][:6]) and all(x > 0 for x in t["tuned_ms"][11:14])
+            timed = [x > 0 for x in t["tuned_ms"][:6] + t["tuned_ms"][11:14]]
+            assert all(timed) if tree == 0 else not any(timed)
         ctx.close()
     for f in FIELDS:
         assert np.array_equal(out[0][f], out[1][f], equal_nan=True), f

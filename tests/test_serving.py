@@ -25,7 +25,7 @@ def test_request_generator_ranges():
 def test_alone_equals_in_mix_under_continuous_batching(precision):
     reqs = serving.make_requests(6, rate=1000.0, seed=11, lo=48, hi=160, steps=4)
     # room for about two requests at a time: later ones join as earlier ones finish
-    srv = serving.Server(200, 200, precision)
+    srv = serving.Server(256, 256, precision)
     live_counts = []
     orig_step = srv.step
 
