@@ -69,6 +69,11 @@ def kernel_bytes(kernel, t):
                distance 4 written, once per cell.
     spill_relax (the Borůvka phases of the spill heights, k_bv_*) is counted
                per basin graph, not per cell: see graph_bytes.
+    In the FP32 form sed_rounds is the engine's first round over every tile
+    (one pass, the bytes above); sed_contract (k_cs_round, the contraction of
+    the cells that round leaves open) is counted per open cell: parameters
+    2t, status word 4 and donor bits 1 read, stone t and status word 4
+    written.
     """
     return {
         "fa_rounds": 2 + t + 8,
@@ -77,6 +82,7 @@ def kernel_bytes(kernel, t):
         "mod1": 8 * t + 8 + 5 * t + 8 + 5 * t,
         "flowdirs": 8 + 1 + 2 + 8 * t,
         "plat_rounds": 8 + 1 + 1 + 2 + 4,
+        "sed_contract": 2 * t + 4 + 1 + t + 4,
     }.get(kernel)
 
 
@@ -88,12 +94,14 @@ def graph_bytes(nedges, nlabels):
 
 
 # kernel-level timing slots (HIP events around exactly these launches)
-KERNEL_SLOTS = ("fa_rounds", "sed_rounds", "mod1", "mod2", "flowdirs", "plat_rounds", "spill_relax")
+KERNEL_SLOTS = ("fa_rounds", "sed_rounds", "mod1", "mod2", "flowdirs", "plat_rounds", "spill_relax", "sed_contract")
 # rocprofv3 kernel names of those slots (the round kernels carry the tuned tile shape)
 def rocprof_name(slot, tuning, t=4):
     tw, th = tuning["dag_tile"]
     ts = "float" if t == 4 else "double"
-    return {"fa_rounds": f"k_dag_round<{tw}, {th}, FaCell", "sed_rounds": f"k_dag_round<{tw}, {th}, SedCell",
+    sw, sh = (32, 32) if t == 4 and tuning.get("fp32_tree_sediment") else (tw, th)   # fixed first-round shape
+    return {"fa_rounds": f"k_dag_round<{tw}, {th}, FaCell", "sed_rounds": f"k_dag_round<{sw}, {sh}, SedCell",
+            "sed_contract": "k_cs_round",
             "mod1": "k_mod1", "mod2": "k_mod2", "flowdirs": "k_flowdirs",
             # pass 2 (raised surface, no slope marks) holds nearly all the rounds
             "plat_rounds": f"k_plat_block<{tuning['plateau_rounds_per_launch']}, {ts}, false>",
@@ -194,7 +202,8 @@ def run_batched(ds, precision, steps, warmup, tuning=None, groups=1):
     summ[0]["tuning"] = atlases[0].tuning()
     # basin graphs of the last step (edge and label counts are per atlas)
     firsts = [a.summary(sl[0]) for a, sl in zip(atlases, slots)]
-    graph = (sum(f["nedges"] for f in firsts), sum(f["nlabels"] for f in firsts))
+    graph = (sum(f["nedges"] for f in firsts), sum(f["nlabels"] for f in firsts),
+             sum(f["sed_open"] for f in firsts))
     for a in atlases:
         a.close()
     return total, lat, prof, summ, dims, graph
@@ -207,12 +216,16 @@ def phase_ms(prof, steps):
 def roofline(prof, steps, cells, t, tuning, graph):
     """Dominant kernel by live HIP-event time; achieved = algorithmic bytes
     per launch / average launch duration (= bytes of all its launches / their
-    total time).  graph: (edges, labels) of the basin graphs of one step."""
+    total time).  graph: (edges, labels) of the basin graphs of one step and
+    the sediment cells the contraction took over."""
     dom = max(KERNEL_SLOTS, key=lambda k: prof[k][0])
     ms, launches = prof[dom]
     if dom == "spill_relax":
-        total_bytes = graph_bytes(*graph) * steps
-        b = graph_bytes(*graph) / cells
+        total_bytes = graph_bytes(*graph[:2]) * steps
+        b = graph_bytes(*graph[:2]) / cells
+    elif dom == "sed_contract":
+        total_bytes = kernel_bytes(dom, t) * graph[2] * steps
+        b = total_bytes / steps / cells
     else:
         b = kernel_bytes(dom, t)
         # the round kernels share one pass over the grid per step; the others
@@ -337,7 +350,7 @@ def main():
                  "phase_ms_per_step": phase_ms(prof64, args.steps),
                  "step_roofline_frac": SIDE * SIDE * args.steps / t64 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9),
                  "roofline": roofline(prof64, args.steps, SIDE * SIDE, 8, s64["tuning"],
-                                      (s64["nedges"], s64["nlabels"])), "tuning": s64["tuning"],
+                                      (s64["nedges"], s64["nlabels"], s64["sed_open"])), "tuning": s64["tuning"],
                  "last_step": {k: s64[k] for k in ("sinkcount", "plateau_rounds1", "plateau_rounds2",
                                                     "fa_uncomputed", "sed_uncomputed")}}
 

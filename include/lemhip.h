@@ -75,6 +75,10 @@ typedef struct {
      * negative (0 < dem < 400), whatever soilT_mode then does with them. */
     int phantom_cells;
     int soilT_negative;
+    /* FP32 form with lem_tuning.fp32_tree_sediment: the cells of the grid's
+     * last sediment pass left open by the engine's first round and routed by
+     * the contraction (0 when the engine ran to the end) */
+    int sed_open;
 } lem_summary;
 
 /* Fields for upload/download (host layout as described above). */
@@ -217,12 +221,13 @@ typedef struct {
      * of rows or the slot count changes); 0 (default): launched directly.
      * Results are identical. */
     int use_graphs;
-    /* FP32 form with single flow directions: 1 (default) routes sediment
-     * with one round of the dependency engine over every 32 x 32 tile (the
-     * cells whose upstream lies inside their tile), then contracts the rest
-     * of the donor forest (unary chains composed as clamped affine functions
-     * of the stone fraction, in synchronous rounds); 0 runs the engine's
-     * rounds to the end as the FP64 form does.  The same cells compute, every
+    /* FP32 form with single flow directions: n = 1..4 (default 3) routes
+     * sediment with n rounds of the dependency engine on a fixed 32 x 32
+     * tiling (the first visits every tile and completes the cells whose
+     * upstream lies inside their tile), then contracts the rest of the donor
+     * forest (unary chains composed as clamped affine functions of the stone
+     * fraction, in synchronous rounds); 0 runs the engine's rounds to the end
+     * as the FP64 form does.  The same cells compute, every
      * field but stone is bit-identical between the two, and stone values
      * differ by a few float ulps where a composed chain reached them.  The
      * tile shape of the first round is fixed (atlas rectangles are placed at
@@ -240,7 +245,9 @@ int lemhip_autotune(lemhip_ctx *ctx);
  * 3 depression routing, 4 runoff kernel, 5 flow accumulation, 6 MOD part 1
  * kernel, 7 sediment routing, 8 MOD part 2 kernel, 9 reductions, 10 step,
  * 11 flow-accumulation round kernels only, 12 sediment round kernels only,
- * 13 plateau round kernels only, 14 spill-height relaxation kernels only.
+ * 13 plateau round kernels only, 14 spill-height kernels only (the Borůvka
+ * phases, without the host reads between them), 15 sediment contraction
+ * round kernels only (slot 12 then holds the engine's first round).
  * enable resets the accumulators; read returns milliseconds and launch
  * counts per slot. */
 int lemhip_profile_enable(lemhip_ctx *ctx, int on);

@@ -889,7 +889,7 @@ extern "C" int lemhip_get_tuning(lemhip_ctx *c, lem_tuning *o)
     for (int k = 0; k < 16; k++) o->tuned_ms[k] = c->tune_ms[k];
     o->fp32_integer_fa = c->fx_enabled ? 1 : 0;
     o->use_graphs = c->use_graphs ? 1 : 0;
-    o->fp32_tree_sediment = c->cs_enabled ? 1 : 0;
+    o->fp32_tree_sediment = c->cs_rounds;
     return LEMHIP_OK;
 }
 
@@ -908,8 +908,8 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     if (k != 4 && k != 8 && k != 16) { c->err = "plateau rounds per launch must be 4, 8 or 16"; return LEMHIP_E_ARG; }
     if (t->fp32_integer_fa != 0 && t->fp32_integer_fa != 1) { c->err = "fp32_integer_fa must be 0 or 1"; return LEMHIP_E_ARG; }
     if (t->use_graphs != 0 && t->use_graphs != 1) { c->err = "use_graphs must be 0 or 1"; return LEMHIP_E_ARG; }
-    if (t->fp32_tree_sediment != 0 && t->fp32_tree_sediment != 1) {
-        c->err = "fp32_tree_sediment must be 0 or 1";
+    if (t->fp32_tree_sediment < 0 || t->fp32_tree_sediment > 4) {
+        c->err = "fp32_tree_sediment must be 0 to 4";
         return LEMHIP_E_ARG;
     }
     c->use_graphs = t->use_graphs != 0;
@@ -920,7 +920,7 @@ extern "C" int lemhip_set_tuning(lemhip_ctx *c, const lem_tuning *t)
     c->dag_batch = t->dag_batch;
     c->plat_k = k;
     c->fx_enabled = t->fp32_integer_fa != 0;
-    c->cs_enabled = t->fp32_tree_sediment != 0;
+    c->cs_rounds = t->fp32_tree_sediment;
     return LEMHIP_OK;
 }
 
@@ -1093,6 +1093,7 @@ extern "C" int lemhip_slot_summary(lemhip_ctx *c, int slot, lem_summary *o)
     o->totwater = h.totwater;
     o->phantom_cells = c->hphantom[slot];
     o->soilT_negative = h.soilT_negative;
+    o->sed_open = c->h_sed_open;
     return LEMHIP_OK;
 }
 

@@ -10,7 +10,7 @@
 // live timing of selected kernels/phases with HIP events on the context stream
 enum {
     PROF_FD = 0, PROF_FLOWDIRS, PROF_PLATEAU, PROF_FLOOD, PROF_RUNOFF, PROF_FA, PROF_MOD1, PROF_SED,
-    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_FA_K, PROF_SED_K, PROF_PLAT_K, PROF_MM_K, PROF_COUNT
+    PROF_MOD2, PROF_REDUCE, PROF_STEP, PROF_FA_K, PROF_SED_K, PROF_PLAT_K, PROF_MM_K, PROF_CS_K, PROF_COUNT
 };
 #define PROF_MAXP 256
 struct ProfSlot {
@@ -154,7 +154,7 @@ struct lemhip_ctx {
     // FP32 sediment routing: the engine's first round, then a contraction of
     // the donor graph (lem_sed32.hip)
     void *cs_rec = nullptr;           // 2 x N unary records, allocated on first use
-    bool cs_enabled = true;           // false: the engine's rounds to the end in the FP32 form under SFD
+    int cs_rounds = 3;                // engine rounds before the contraction (0: the engine to the end)
     // MOD's straight-line segments (part 1 with its totals; the totals,
     // part 2 and the outlet bookkeeping) replayed from HIP graphs
     bool use_graphs = false;
@@ -169,6 +169,7 @@ struct lemhip_ctx {
     // counters known on the host after each phase
     int h_rounds1 = 0, h_rounds2 = 0, h_nlab = 0, h_nedges = 0, h_sinkcount = 0, h_sinks1 = 0;
     int h_fa_unc = 0, h_sed_unc = 0;
+    int h_sed_open = 0;   // sediment cells the contraction took over (last pass)
 
     // profiling
     bool prof_on = false;
@@ -227,7 +228,7 @@ int lem_runoff_phase(lemhip_ctx *c);
 int lem_calcwater(lemhip_ctx *c);
 int lem_fa32_sfd(lemhip_ctx *c, int *fallback);
 int lem_sed32(lemhip_ctx *c, int *fallback);
-int lem_sed_first(lemhip_ctx *c);
+int lem_sed_first(lemhip_ctx *c, int rounds);
 int lem_fa_phase(lemhip_ctx *c, int iter);
 int lem_mod_phase(lemhip_ctx *c, int iter);
 int lem_sync_scalars(lemhip_ctx *c);

@@ -809,10 +809,10 @@ __global__ void k_iota(int n, int *a)
     if (i < n) a[i] = i;
 }
 
-// one_launch: only the first round (every tile once), with no wait for the
-// device; *uncomputed is not set
+// first_rounds > 0: only that many rounds (the first visits every tile),
+// with no wait for the device; *uncomputed is not set
 template <int TW, int TH, class Cell, bool SFD>
-static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot, bool one_launch = false)
+static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot, int first_rounds = 0)
 {
     int ntiles = ((c->W + TW - 1) / TW) * ((c->H + TH - 1) / TH);
     int *st = reinterpret_cast<int *>(c->scan_tmp);
@@ -835,14 +835,16 @@ static int run_rounds(lemhip_ctx *c, Cell cell, int *uncomputed, int prof_slot, 
         bpc = 4;
     int G = bpc * c->ncu;
     if (G > ntiles) G = ntiles;
-    if (one_launch) {
-        ++r;
+    if (first_rounds > 0) {
         prof_begin(c, prof_slot);
-        k_dag_round<TW, TH, Cell, SFD><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, L[r % 3],
-                                                       L[(r + 1) % 3], c->dag_wake, ctl);
-        prof_end(c, prof_slot, 1);
+        for (int q = 0; q < first_rounds; q++) {
+            ++r;
+            k_dag_round<TW, TH, Cell, SFD><<<G, 256, 0, c->s>>>(c->W, c->H, r, cell, c->dmask, c->fd, st, L[r % 3],
+                                                           L[(r + 1) % 3], c->dag_wake, ctl);
+        }
+        prof_end(c, prof_slot, first_rounds);
         LEM_LAUNCHED(c);
-        c->dag_rounds_last = 1;
+        c->dag_rounds_last = first_rounds;
         c->dag_round = r;
         return LEMHIP_OK;
     }
@@ -936,6 +938,7 @@ int lem_sed_prep(lemhip_ctx *c, int *bad)
 template <typename T>
 int lem_sed_dag(lemhip_ctx *c)
 {
+    c->h_sed_open = 0;
     LEM_TRY(lem_sed_prep<T>(c, nullptr));
     switch (c->dag_shape) {
 #define X(id, tw, th) case id: return sed_shape<T, tw, th>(c);
@@ -945,20 +948,21 @@ int lem_sed_dag(lemhip_ctx *c)
     }
 }
 
-// The first round of the engine's sediment pass in the FP32 form under
-// single flow directions, after lem_sed_prep: every 32 x 32 tile once,
-// completing each cell whose donors all lie upstream inside its tile (on
-// the fBm workloads about 80% of the cells).  Cells with a donor path
-// entering from another tile stay pending for the contraction
-// (lem_sed32.hip).  The tile shape is fixed, so which cells complete here
-// depends on the cell's position modulo 32 only.
-int lem_sed_first(lemhip_ctx *c)
+// The first rounds of the engine's sediment pass in the FP32 form under
+// single flow directions, after lem_sed_prep.  Round 1 visits every 32 x 32
+// tile and completes each cell whose upstream lies inside its tile (on the
+// fBm workloads about 82% of the cells); each later round visits the tiles
+// woken by the last one's exits and completes the cells whose upstream
+// crosses into them only from cells completed before (after round 2 about
+// 91%).  The rest stays pending for the contraction (lem_sed32.hip).  The tile shape is fixed, so
+// which cells complete here depends on the cell's position modulo 32 only.
+int lem_sed_first(lemhip_ctx *c, int rounds)
 {
     SedCell<float, 32, 32> cell{reinterpret_cast<const float *>(c->ero), reinterpret_cast<const float *>(c->inci),
                                 reinterpret_cast<const float *>(c->geli), reinterpret_cast<const float *>(c->z2),
                                 reinterpret_cast<float *>(c->depo), reinterpret_cast<float *>(c->stone),
                                 reinterpret_cast<float *>(c->fines), c->soilT};
-    return run_rounds<32, 32, decltype(cell), true>(c, cell, nullptr, PROF_SED_K, true);
+    return run_rounds<32, 32, decltype(cell), true>(c, cell, nullptr, PROF_SED_K, rounds);
 }
 
 // Times flow accumulation with every tile shape, then the best shape with

@@ -1399,36 +1399,38 @@ static int spill_heights(lemhip_ctx *c, int nlab, int ne)
     const uint32_t *lo = c->elo, *hi = c->ehi;
     const unsigned long long *w = c->ew;
     int na = ne, buf = 0;
-    prof_begin(c, PROF_MM_K);
-    long launches = 0;
+    // the profile slot times the kernels only, not the host reads between them
     for (int ph = 0; na > 0; ph++) {
         const int p = ph & 1;
         const int ge = grid1d(na, 256, 1 << 30);
+        prof_begin(c, PROF_MM_K);
         k_bv_min_w<<<ge, 256, 0, c->s>>>(na, lo, hi, w, c->comp, bw[p]);
         k_bv_min_e<<<ge, 256, 0, c->s>>>(na, lo, hi, w, c->comp, bw[p], be[p]);
         k_bv_hook<<<grid1d(n), 256, 0, c->s>>>(n, lo, hi, c->comp, be[p], nxt);
         k_bv_break<<<grid1d(n), 256, 0, c->s>>>(n, be[p], nxt);
+        prof_end(c, PROF_MM_K, 4);
         LEM_LAUNCHED(c);
-        launches += 4;
         for (int it = 0;; it++) {
             LEM_TRY(clear_flag(c));
+            prof_begin(c, PROF_MM_K);
             for (int q = 0; q < 4; q++) k_bv_jump<<<grid1d(n), 256, 0, c->s>>>(n, nxt, &c->scal->flag);
+            prof_end(c, PROF_MM_K, 4);
             LEM_LAUNCHED(c);
-            launches += 4;
             int f = 0;
             LEM_TRY(read_flag(c, &f));
             if (!f) break;
             if (it > 16) { c->err = "spill-height phases did not converge"; return LEMHIP_E_NOT_CONVERGED; }
         }
-        k_bv_root0<<<1, 1, 0, c->s>>>(c->comp, bw[p], be[p], nxt, st);
-        k_bv_labels<<<grid1d(n), 256, 0, c->s>>>(n, c->comp, bw[p], be[p], nxt, mv, c->rmm, st, bw[p ^ 1],
-                                                  be[p ^ 1]);
         uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
         unsigned long long *w2 = c->ew2[buf];
         LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
+        prof_begin(c, PROF_MM_K);
+        k_bv_root0<<<1, 1, 0, c->s>>>(c->comp, bw[p], be[p], nxt, st);
+        k_bv_labels<<<grid1d(n), 256, 0, c->s>>>(n, c->comp, bw[p], be[p], nxt, mv, c->rmm, st, bw[p ^ 1],
+                                                  be[p ^ 1]);
         k_bv_keep<<<grid_chunks(na), 256, 0, c->s>>>(na, lo, hi, w, c->comp, lo2, hi2, w2, &c->scal->nedges);
+        prof_end(c, PROF_MM_K, 3);
         LEM_LAUNCHED(c);
-        launches += 3;
         LEM_TRY(read_scalars(c));
         const int nk = c->hscal->nedges;
         // every component with an outgoing edge joins another each phase, so
@@ -1439,7 +1441,6 @@ static int spill_heights(lemhip_ctx *c, int nlab, int ne)
         buf ^= 1;
         if (ph > 64) { c->err = "spill-height phases did not converge"; return LEMHIP_E_NOT_CONVERGED; }
     }
-    prof_end(c, PROF_MM_K, launches);
     return LEMHIP_OK;
 }
 

@@ -748,13 +748,13 @@ static int mod_t(lemhip_ctx *c)
     if (c->use_graphs) prof_end(c, PROF_MOD1);
     if (c->tune_dag_pending) {
         // still pending only in the FP32 form with integer flow accumulation
-        if (sizeof(T) == 4 && !(c->p.flowdirectiontype == 0 && c->cs_enabled)) LEM_TRY(lem_tune_sed(c));
+        if (sizeof(T) == 4 && !(c->p.flowdirectiontype == 0 && c->cs_rounds > 0)) LEM_TRY(lem_tune_sed(c));
         c->tune_dag_pending = false;
     }
     prof_begin(c, PROF_SED);
     {
         int fallback = 1;
-        if (sizeof(T) == 4 && c->p.flowdirectiontype == 0 && c->cs_enabled) LEM_TRY(lem_sed32(c, &fallback));
+        if (sizeof(T) == 4 && c->p.flowdirectiontype == 0 && c->cs_rounds > 0) LEM_TRY(lem_sed32(c, &fallback));
         if (fallback) LEM_TRY(lem_sed_dag<T>(c));
     }
     prof_end(c, PROF_SED, c->dag_rounds_last);

@@ -132,8 +132,8 @@ __device__ __forceinline__ void cs_chunk_put(CsChunk &C, int j, bool keep, int c
 }
 
 // reserve the chunk's range with one atomic, then write the kept entries in
-// row order
-__device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count)
+// row order (to a second list as well when copy is given)
+__device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count, int *copy = nullptr)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -149,8 +149,11 @@ __device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int j = 0; j < CS_ITEMS; j++) {
         const unsigned long long b = C.bal[j][wave];
-        if ((b >> lane) & 1ull)
-            list[C.base + C.off[j][wave] + __popcll(b & ((1ull << lane) - 1ull))] = C.cell[j * 256 + threadIdx.x];
+        if ((b >> lane) & 1ull) {
+            const int at = C.base + C.off[j][wave] + __popcll(b & ((1ull << lane) - 1ull));
+            list[at] = C.cell[j * 256 + threadIdx.x];
+            if (copy) copy[at] = C.cell[j * 256 + threadIdx.x];
+        }
     }
     __syncthreads();
 }
@@ -159,16 +162,18 @@ __device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count
 // completed is done and one it killed (or the prep kernel marked dead) is
 // dead, both with stamp 0; a pending cell gets its rake parameters, as the
 // engine's cell body forms them from its unchanged inputs, and is open.
-// The open cells form round 2's list.  Each thread rewrites only its own
+// The open cells form round 2's list (and its copy for k_cs_finish); the
+// dead cells are counted as uncomputed.  Each thread rewrites only its own
 // cells' words.
 __global__ void __launch_bounds__(256) k_cs_from_engine(long N, const uint8_t *__restrict__ mask, int *st,
                                                         const double *__restrict__ soilT,
                                                         const float *__restrict__ dps,
                                                         const float *__restrict__ stone, float *ca, float *cb,
-                                                        int *list, CsCtl *ctl)
+                                                        int *list, int *list0, CsCtl *ctl)
 {
     __shared__ CsChunk C;
     const long CH = 256L * CS_ITEMS;
+    int unc = 0;
     for (long c0 = (long)blockIdx.x * CH; c0 < N; c0 += (long)gridDim.x * CH) {
 #pragma unroll 1
         for (int j = 0; j < CS_ITEMS; j++) {
@@ -196,13 +201,17 @@ __global__ void __launch_bounds__(256) k_cs_from_engine(long N, const uint8_t *_
                     keep = true;
                 } else if (m && s < 0) {
                     w = cs_word(CS_DEAD, CS_DEAD, 0, 0);
+                    unc++;
                 }
                 st[i] = (int)w;
             }
             cs_chunk_put(C, j, keep, (int)i);
         }
-        cs_chunk_flush(C, list, &ctl->cnt[2]);
+        cs_chunk_flush(C, list, &ctl->cnt[2], list0);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) unc += __shfl_down(unc, o);
+    if ((threadIdx.x & 63) == 0 && unc) atomicAdd(&ctl->unc, unc);
 }
 
 // One synchronous round r >= 2 over the open cells (list of round r:
@@ -297,24 +306,23 @@ __global__ void __launch_bounds__(256) k_cs_round(int W, uint32_t r, const int *
     }
 }
 
-// Results of the cells computed by the rounds as the engine's cell body
-// writes them (deposition, fines, soil thickness; stone is already in
-// place); the engine's first round has written its own cells (done with
-// stamp 0); every other mask cell is left as it was and counted.
-__global__ void __launch_bounds__(256) k_cs_finish(long N, const uint8_t *__restrict__ mask,
+// Results of the cells the rounds computed (the open cells of round 2's
+// list), as the engine's cell body writes them (deposition, fines, soil
+// thickness; stone is already in place); the engine's rounds have written
+// their own cells; an open cell that never computed is left as it was and
+// counted.
+__global__ void __launch_bounds__(256) k_cs_finish(int n, const int *__restrict__ list0,
                                                    const uint32_t *__restrict__ stw, const float *__restrict__ dps,
                                                    const float *__restrict__ ero, const float *__restrict__ inci,
                                                    const float *__restrict__ geli, const float *__restrict__ stone,
                                                    float *depo, float *fines, double *soilT, CsCtl *ctl)
 {
     int unc = 0;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        const uint8_t m = mask[i];
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const long i = list0[k];
         const uint32_t w = stw[i];
-        if (!m) continue;
         if ((w & 3u) != CS_DONE) { unc++; continue; }
-        if ((w >> 5) == 0) continue;
         const float dp = dps[i];
         const double sT = soilT[i];
         const double sedthick = sT + (double)dp;
@@ -365,29 +373,34 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
     *fallback = 0;
     // round 1: the engine's first round over every tile (its control block
     // shares the memory of this one)
-    LEM_TRY(lem_sed_first(c));
+    LEM_TRY(lem_sed_first(c, c->cs_rounds));
     const long CH = 256L * CS_ITEMS;
     auto chunks = [&](long n) { return (int)std::max(1L, std::min((n + CH - 1) / CH, 4096L)); };
     LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(CsCtl), c->s));
-    // status words and round 2's list (lists[0], count in cnt[2])
+    // status words and round 2's list (lists[0], count in cnt[2]); its copy
+    // for k_cs_finish in the upper half of the raised-surface buffer (dps
+    // holds the lower half as floats)
+    int *list0 = reinterpret_cast<int *>(c->z2) + NP;
     k_cs_from_engine<<<chunks(N), 256, 0, c->s>>>(N, c->mask, c->scan_tmp, c->soilT, dps, (const float *)c->stone,
-                                                  ca, cb, lists[0], ctl);
+                                                  ca, cb, lists[0], list0, ctl);
     LEM_LAUNCHED(c);
     LEM_CHECK(c, hipMemcpyAsync(&c->hint[24], &ctl->cnt[2], sizeof(int), hipMemcpyDeviceToHost, c->s));
     LEM_CHECK(c, hipStreamSynchronize(c->s));
     long open = c->hint[24];
+    const long open0 = open;
+    c->h_sed_open = (int)open;
     uint32_t r = 1;
     long launches = 1;
     const int BATCH = 4;
     while (open > 0) {
-        prof_begin(c, PROF_SED_K);
+        prof_begin(c, PROF_CS_K);
         for (int q = 0; q < BATCH; q++) {
             ++r;
             k_cs_round<<<chunks(open), 256, 0, c->s>>>(c->W, r, lists[r & 1], lists[(r + 1) & 1], ctl, stw, c->dmask,
                                                        (float *)c->stone, ca, cb, rec[(r - 1) & 1], rec[r & 1],
                                                        c->demid, c->ddesc);
         }
-        prof_end(c, PROF_SED_K, BATCH);
+        prof_end(c, PROF_CS_K, BATCH);
         LEM_LAUNCHED(c);
         launches += BATCH;
         LEM_CHECK(c, hipMemcpyAsync(&c->hint[24], &ctl->cnt[(r + 1) % 3], sizeof(int), hipMemcpyDeviceToHost,
@@ -396,7 +409,7 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
         open = c->hint[24];
         if (r > (1u << 20)) { c->err = "sediment contraction did not converge"; return LEMHIP_E_NOT_CONVERGED; }
     }
-    k_cs_finish<<<lem_grid1d(N), 256, 0, c->s>>>(N, c->mask, stw, dps, (const float *)c->ero,
+    k_cs_finish<<<lem_grid1d(std::max(open0, 1L)), 256, 0, c->s>>>((int)open0, list0, stw, dps, (const float *)c->ero,
                                                   (const float *)c->inci, (const float *)c->geli,
                                                   (const float *)c->stone, (float *)c->depo, (float *)c->fines,
                                                   c->soilT, ctl);

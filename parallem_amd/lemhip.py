@@ -47,7 +47,8 @@ EXPORTS = [
 ]
 
 PROF_SLOTS = ["fd", "flowdirs", "plateau", "depression", "runoff", "flow_accum", "mod1", "sediment",
-              "mod2", "reductions", "step", "fa_rounds", "sed_rounds", "plat_rounds", "spill_relax"]
+              "mod2", "reductions", "step", "fa_rounds", "sed_rounds", "plat_rounds", "spill_relax",
+              "sed_contract"]
 
 
 class LemTuning(C.Structure):
@@ -90,6 +91,7 @@ class LemSummary(C.Structure):
         ("nlabels", C.c_int), ("nedges", C.c_int), ("plateau_rounds1", C.c_int),
         ("plateau_rounds2", C.c_int), ("fa_uncomputed", C.c_int), ("sed_uncomputed", C.c_int),
         ("totwater", C.c_double), ("phantom_cells", C.c_int), ("soilT_negative", C.c_int),
+        ("sed_open", C.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -241,10 +241,11 @@ def test_integer_accumulation_equals_the_engine():
     assert ctx[1].tuning()["fp32_integer_fa"] == 1 and ctx[0].tuning()["fp32_integer_fa"] == 0
 
 
-def test_tree_sediment_equals_the_engine():
+@pytest.mark.parametrize("rounds", [1, 3])
+def test_tree_sediment_equals_the_engine(rounds):
     """FP32 form, single flow directions: sediment routing by the engine's
-    first round plus a contraction of the rest of the donor forest (the
-    default) against the engine's rounds to the end, one erosion phase at a
+    first rounds plus a contraction of the rest of the donor forest (the
+    default, with three rounds) against the engine's rounds to the end, one erosion phase at a
     time from identical state, the outlet cycle included (runiniter = 0).
     The same cells compute; deposition, soil thickness and every cell whose
     stone value is reached without a composed chain are bit-identical; stone
@@ -255,7 +256,7 @@ def test_tree_sediment_equals_the_engine():
     for tree in (1, 0):
         c = lemhip.LemContext(lemhip.default_params(W, H, 0, runiniter=0), lemhip.FP32_KAHAN, 0)
         c.init_state(dem.z, dem.rain, soilT_mode="nonneg")
-        c.set_tuning(fp32_tree_sediment=tree)
+        c.set_tuning(fp32_tree_sediment=rounds if tree else 0)
         ctx[tree] = c
     m = ctx[0].download("mask") == 1
     for k in range(3):
@@ -275,12 +276,13 @@ def test_tree_sediment_equals_the_engine():
         a, b = t.download("stone")[m], e.download("stone")[m]
         rel = float(np.max(np.abs(a.astype(np.float64) - b) / np.abs(b)))
         same = float(np.mean(a == b))
-        print(f"\n[fp32] tree vs engine sediment, step {k}: stone max rel {rel:.2e}, bit-equal {same:.4f}, "
+        print(f"\n[fp32] tree ({rounds} engine rounds) vs engine sediment, step {k}: stone max rel {rel:.2e}, "
+              f"bit-equal {same:.4f}, "
               f"uncomputed {t.summary()['sed_uncomputed']}")
         assert rel <= 1e-6
         fa_, fb_ = t.download("fines")[m], e.download("fines")[m]
         assert float(np.max(np.abs(fa_ - fb_))) <= 1e-4
-    assert ctx[1].tuning()["fp32_tree_sediment"] == 1 and ctx[0].tuning()["fp32_tree_sediment"] == 0
+    assert ctx[1].tuning()["fp32_tree_sediment"] == rounds and ctx[0].tuning()["fp32_tree_sediment"] == 0
 
 
 def test_integer_accumulation_falls_back_on_negative_weights():
