@@ -111,20 +111,28 @@ __device__ __forceinline__ CsRec cs_compose(const CsRec &f, const CsRec &g)
     return h;
 }
 
-// Open cells are kept in lists built in chunks of 256 * CS_ITEMS entries per
+// Open cells are kept in lists built in chunks of 256 * IT entries per
 // workgroup: one global atomic reserves a chunk's output range (workgroups
 // adding to one counter serialise, so one atomic per wave would dominate).
+// The pass over every cell takes 16 rows per workgroup; a round takes 2, so
+// that more workgroups share the gathers of a short list (a thread works
+// through its rows one after another).
 #define CS_ITEMS 16
+#ifndef CS_ROUND_ITEMS
+#define CS_ROUND_ITEMS 2
+#endif
 
+template <int IT>
 struct CsChunk {
-    int cell[CS_ITEMS * 256];
-    unsigned long long bal[CS_ITEMS][4];
-    int off[CS_ITEMS][4];
+    int cell[IT * 256];
+    unsigned long long bal[IT][4];
+    int off[IT][4];
     int base;
 };
 
 // row j of the chunk: record this thread's entry and the wave's keep mask
-__device__ __forceinline__ void cs_chunk_put(CsChunk &C, int j, bool keep, int cell)
+template <int IT>
+__device__ __forceinline__ void cs_chunk_put(CsChunk<IT> &C, int j, bool keep, int cell)
 {
     const unsigned long long b = __ballot(keep);
     if ((threadIdx.x & 63) == 0) C.bal[j][threadIdx.x >> 6] = b;
@@ -133,12 +141,13 @@ __device__ __forceinline__ void cs_chunk_put(CsChunk &C, int j, bool keep, int c
 
 // reserve the chunk's range with one atomic, then write the kept entries in
 // row order (to a second list as well when copy is given)
-__device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count, int *copy = nullptr)
+template <int IT>
+__device__ __forceinline__ void cs_chunk_flush(CsChunk<IT> &C, int *list, int *count, int *copy = nullptr)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
         int tot = 0;
-        for (int j = 0; j < CS_ITEMS; j++)
+        for (int j = 0; j < IT; j++)
             for (int w = 0; w < 4; w++) {
                 C.off[j][w] = tot;
                 tot += __popcll(C.bal[j][w]);
@@ -147,7 +156,7 @@ __device__ __forceinline__ void cs_chunk_flush(CsChunk &C, int *list, int *count
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = 0; j < CS_ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         const unsigned long long b = C.bal[j][wave];
         if ((b >> lane) & 1ull) {
             const int at = C.base + C.off[j][wave] + __popcll(b & ((1ull << lane) - 1ull));
@@ -171,7 +180,7 @@ __global__ void __launch_bounds__(256) k_cs_from_engine(long N, const uint8_t *_
                                                         const float *__restrict__ stone, float *ca, float *cb,
                                                         int *list, int *list0, CsCtl *ctl)
 {
-    __shared__ CsChunk C;
+    __shared__ CsChunk<CS_ITEMS> C;
     const long CH = 256L * CS_ITEMS;
     int unc = 0;
     for (long c0 = (long)blockIdx.x * CH; c0 < N; c0 += (long)gridDim.x * CH) {
@@ -224,14 +233,14 @@ __global__ void __launch_bounds__(256) k_cs_round(int W, uint32_t r, const int *
                                                   const float *__restrict__ cb, const CsRec *rin, CsRec *rout,
                                                   const uint8_t *__restrict__ demid, const DemDesc *__restrict__ dd)
 {
-    __shared__ CsChunk C;
+    __shared__ CsChunk<CS_ROUND_ITEMS> C;
     const int n = ctl->cnt[r % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cnt[(r + 2) % 3] = 0;
-    const long CH = 256L * CS_ITEMS;
+    const long CH = 256L * CS_ROUND_ITEMS;
     const uint32_t prev_round = r - 1;
     for (long c0 = (long)blockIdx.x * CH; c0 < n; c0 += (long)gridDim.x * CH) {
 #pragma unroll 1
-        for (int j = 0; j < CS_ITEMS; j++) {
+        for (int j = 0; j < CS_ROUND_ITEMS; j++) {
             const long k = c0 + j * 256 + threadIdx.x;
             bool keep = false;
             int i = 0;
@@ -376,6 +385,8 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
     LEM_TRY(lem_sed_first(c, c->cs_rounds));
     const long CH = 256L * CS_ITEMS;
     auto chunks = [&](long n) { return (int)std::max(1L, std::min((n + CH - 1) / CH, 4096L)); };
+    const long CHR = 256L * CS_ROUND_ITEMS;
+    auto chunks_r = [&](long n) { return (int)std::max(1L, std::min((n + CHR - 1) / CHR, 16384L)); };
     LEM_CHECK(c, hipMemsetAsync(ctl, 0, sizeof(CsCtl), c->s));
     // status words and round 2's list (lists[0], count in cnt[2]); its copy
     // for k_cs_finish in the upper half of the raised-surface buffer (dps
@@ -396,7 +407,7 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
         prof_begin(c, PROF_CS_K);
         for (int q = 0; q < BATCH; q++) {
             ++r;
-            k_cs_round<<<chunks(open), 256, 0, c->s>>>(c->W, r, lists[r & 1], lists[(r + 1) & 1], ctl, stw, c->dmask,
+            k_cs_round<<<chunks_r(open), 256, 0, c->s>>>(c->W, r, lists[r & 1], lists[(r + 1) & 1], ctl, stw, c->dmask,
                                                        (float *)c->stone, ca, cb, rec[(r - 1) & 1], rec[r & 1],
                                                        c->demid, c->ddesc);
         }
