@@ -734,6 +734,21 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
             __syncthreads();
         }
         if (my_exits) atomicOr(&exits, my_exits);
+        __syncthreads();
+        // wake each neighbour the chains flow into, once per round: gathered
+        // in LDS and appended to the next list with one atomic per full
+        // buffer (workgroups adding to one counter serialise).  Before the
+        // HBM writes below: the wait for the atomic's returned value would
+        // otherwise also wait for those stores (the vector memory counter
+        // counts stores as well), and the next visit's loads overlap them
+        if (threadIdx.x < 9 && ((exits >> threadIdx.x) & 1)) {
+            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+            if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
+                const int t2 = ty * ntx + tx;
+                // not queued already, nor finished
+                if (atomicMax(&wake[t2], round + 1) < round + 1) qbuf[atomicAdd(&qn, 1)] = t2;
+            }
+        }
         // 4. HBM writes of the cells resolved in this round, and bookkeeping
         bool pend = false;
 #pragma unroll
@@ -754,17 +769,6 @@ __global__ void __launch_bounds__(256, Cell::kWaves) k_dag_round(int W, int H, i
         }
         if (pend) any_pending = 1;
         int prog = __syncthreads_or(seeds != 0);
-        // wake each neighbour the chains flow into, once per round: gathered
-        // in LDS and appended to the next list with one atomic per full
-        // buffer (workgroups adding to one counter serialise)
-        if (threadIdx.x < 9 && ((exits >> threadIdx.x) & 1)) {
-            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
-            if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
-                const int t2 = ty * ntx + tx;
-                // not queued already, nor finished
-                if (atomicMax(&wake[t2], round + 1) < round + 1) qbuf[atomicAdd(&qn, 1)] = t2;
-            }
-        }
         if (threadIdx.x == 0) {
             if (prog) wg_prog = true;
             if (!any_pending) atomicExch(&wake[tile], DAG_TILE_DONE);
