@@ -535,6 +535,19 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
             if (!any) break;
         }
         if (moved) atomicMax(&last, moved);
+        __syncthreads();
+        // still moving at the end of the launch: this tile and its neighbours
+        // go on, gathered in LDS and appended to the next list with one
+        // atomic per full buffer (workgroups adding to one counter serialise).
+        // Before the write-back: the wait for the atomic's returned value
+        // would otherwise also wait for those stores
+        if (last == K && threadIdx.x < 9) {
+            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
+            if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
+                const int t2 = ty * ntx + tx;
+                if (tile_has[t2] && atomicExch(&tstamp[t2], launch + 1) != launch + 1) qbuf[atomicAdd(&qn, 1)] = t2;
+            }
+        }
         // write back the tile: distances (into its other buffer), directions of
         // changed cells, slope marks (pass 1 only)
         float *sp_out = s_buf[4] ? sp0 : sp1;
@@ -558,16 +571,6 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
         if (threadIdx.x == 0) {
             tver[tile] = (launch << 1) | (s_buf[4] ? 0 : 1);
             if (last > 0 && r0 + last > wg_last) wg_last = r0 + last;
-        }
-        // still moving at the end of the launch: this tile and its neighbours
-        // go on, gathered in LDS and appended to the next list with one
-        // atomic per full buffer (workgroups adding to one counter serialise)
-        if (last == K && threadIdx.x < 9) {
-            int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
-            if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
-                const int t2 = ty * ntx + tx;
-                if (tile_has[t2] && atomicExch(&tstamp[t2], launch + 1) != launch + 1) qbuf[atomicAdd(&qn, 1)] = t2;
-            }
         }
         __syncthreads();
         if (qn > PLAT_QCAP - 9) plat_flush(qbuf, &qn, next, &ctl->cnt[(launch + 1) % 3]);
