@@ -53,3 +53,28 @@ def test_bad_arguments_are_status_codes_not_crashes():
     p = lemhip.default_params(10, 10)
     assert L.lemhip_create(ctypes.byref(p), 7, 0, ctypes.byref(h)) == -1
     assert L.lemhip_step(None, 0) == -1
+
+
+def test_atlas_span_rounds_rectangles_to_the_alignment():
+    """Rectangles start at multiples of ATLAS_ALIGN: each DEM but the last
+    takes its width plus the nodata column rounded up, the last only its
+    width plus the nodata column."""
+    from parallem_amd import lemhip
+    assert lemhip.ATLAS_ALIGN == 32
+    assert lemhip.atlas_span([]) == 0
+    assert lemhip.atlas_span([100]) == 101
+    assert lemhip.atlas_span([4096, 4096]) == 4128 + 4097
+    assert lemhip.atlas_span([31, 31, 31]) == 32 + 32 + 32
+    assert lemhip.atlas_span([32, 5]) == 64 + 6
+
+
+def test_summary_and_profile_layouts_match_the_header():
+    """The Python mirrors of lem_summary and the profile slots end where the
+    header's do (a field added on one side only would shift every later one)."""
+    from parallem_amd import lemhip
+    hdr = open(os.path.join(ROOT, "include", "lemhip.h")).read()
+    body = hdr[hdr.index("typedef struct {\n    double FA_max"):hdr.index("} lem_summary;")]
+    last = [l.strip() for l in body.splitlines() if l.strip().startswith("int ") or l.strip().startswith("double ")][-1]
+    assert last == "int sed_open;"
+    assert lemhip.LemSummary._fields_[-1][0] == "sed_open"
+    assert len(lemhip.PROF_SLOTS) == 16 and lemhip.PROF_SLOTS[-1] == "sed_contract"
