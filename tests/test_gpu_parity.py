@@ -226,3 +226,31 @@ def test_temperature_change_mid_run():
         ctx.step(it + k); orc.step(it + k)
         _assert_fields(ctx, orc, f"after set_climate, step {k}", fields=["dem", "soilT", "stone", "wC", "wP"])
     _assert_summary(ctx, orc, "final")
+
+
+@pytest.mark.parametrize("flowtype", [0, 1])
+def test_long_chain_of_basins(flowtype):
+    """A valley of ~100 one-cell pits whose sills rise away from the grid
+    edge, with stretches of equal sills: each basin's lightest spill leads
+    to its downstream neighbour, so the spill heights come from chains of
+    basins joining in one phase (many pointer-jumping passes) and from tied
+    weights.  Directions, raised surface and everything downstream match
+    the oracle's literal Kruskal pass bit for bit over three steps."""
+    H, W = 25, 640
+    y = np.arange(H)[:, None].astype(np.float64)
+    x = np.arange(W)[None, :].astype(np.float64)
+    floor = 100.0 + 0.05 * np.floor(x / 40.0) * 40.0 + 0.01 * x * ((x // 80) % 2)   # rising, with level stretches
+    z = floor + 2.0 * np.abs(y - 12.0)
+    for px in range(8, W - 4, 6):
+        z[12, px] -= 3.0
+    z[0, :] = -9999.0
+    z[-1, :] = -9999.0
+    z[:, -1] = -9999.0
+    dem = synth.Dem(z=z, rain=np.full(z.shape, 1000.0), seed=0)
+    ctx, orc = _pair(dem, flowtype)
+    for k in range(3):
+        ctx.step(-1000 + k)
+        orc.step(-1000 + k)
+        _assert_fields(ctx, orc, f"step {k}", fields=["dem", "fa", "soilT", "stone", "depo"])
+        _assert_summary(ctx, orc, f"step {k}")
+    assert ctx.summary()["nlabels"] > 90
