@@ -8,7 +8,7 @@ unclamped formula turns every cell below 400 m into NaN after one step):
 * value — batched FP32 + compensated accumulation: B (default 8) independent
   4096 x 4096 fBm DEMs (diamond-square H = 0.8, 0-800 m, 0.5 % tilt, 400 pits,
   100 plateaus, 5 % nodata frame, per-cell lognormal(0, 0.3) rain rescaled to
-  a 1000 mm/yr mean; seeds differ per DEM) split over --groups (default 2)
+  a 1000 mm/yr mean; seeds differ per DEM) split over --groups (default 3)
   atlas grids, stepped together on one stream each;
 * single_fp32 — one such DEM alone, same precision form;
 * plain_fp64 — one such DEM alone in the plain FP64 form (bit-exact with the
@@ -298,7 +298,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=8, help="4096^2 DEMs in the batched form")
-    ap.add_argument("--groups", type=int, default=2,
+    ap.add_argument("--groups", type=int, default=3,
                     help="atlases the batch is split over, stepped together on their own streams")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-DEM forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
