@@ -1021,25 +1021,41 @@ __global__ void k_bv_break(long n, const uint32_t *__restrict__ be, uint32_t *nx
     }
 }
 
-// pointer jumping to the roots of the joined components
-__global__ void k_bv_jump(long n, uint32_t *nxt, int *flag)
+// pointer jumping towards the roots of the joined components (a fixed
+// number of passes; the readers below walk what is left)
+__global__ void k_bv_jump(long n, uint32_t *nxt)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const uint32_t t = nxt[i];
         const uint32_t tt = nxt[t];
-        const bool ch = tt != t;
-        if (ch) nxt[i] = tt;
-        wave_flag(ch, flag);
+        if (tt != t) nxt[i] = tt;
     }
+}
+
+// The root of x's joined component: the pointers form trees whose roots
+// point at themselves (the only cycles Borůvka's choices make under a
+// strict order are pairs, and k_bv_break turns those into roots), and after
+// the jumping passes few links are left.  A walk longer than any tree could
+// be raises the error flag instead of running on.
+#define BV_WALK_MAX (1u << 20)
+__device__ __forceinline__ uint32_t bv_root(const uint32_t *nxt, uint32_t x, int *err)
+{
+    for (uint32_t h = 0; h < BV_WALK_MAX; h++) {
+        const uint32_t y = nxt[x];
+        if (y == x) return x;
+        x = y;
+    }
+    *err = 1;
+    return x;
 }
 
 // label 0's running maximum and its new component: st[0], st[1]
 __global__ void k_bv_root0(const uint32_t *comp, const unsigned long long *bw, const uint32_t *be,
-                           const uint32_t *nxt, unsigned long long *st)
+                           const uint32_t *nxt, unsigned long long *st, int *err)
 {
     const uint32_t c0 = comp[0];
     if (be[c0] != ~0u && bw[c0] > st[0]) st[0] = bw[c0];
-    st[1] = nxt[c0];
+    st[1] = bv_root(nxt, nxt[c0], err);
 }
 
 // every label: its running maximum, its new component, and its spill height
@@ -1048,7 +1064,7 @@ __global__ void k_bv_root0(const uint32_t *comp, const unsigned long long *bw, c
 __global__ void k_bv_labels(long n, uint32_t *comp, const unsigned long long *__restrict__ bw,
                             const uint32_t *__restrict__ be, const uint32_t *__restrict__ nxt,
                             unsigned long long *mv, unsigned long long *r, const unsigned long long *st,
-                            unsigned long long *bwn, uint32_t *ben)
+                            unsigned long long *bwn, uint32_t *ben, int *err)
 {
     const unsigned long long INF = ord_enc(lm_inf()), NEG = ord_enc(-lm_inf());
     const unsigned long long m0 = st[0];
@@ -1058,7 +1074,7 @@ __global__ void k_bv_labels(long n, uint32_t *comp, const unsigned long long *__
         unsigned long long m = mv[i];
         if (be[c] != ~0u && bw[c] > m) m = bw[c];
         mv[i] = m;
-        const uint32_t nc = nxt[c];
+        const uint32_t nc = bv_root(nxt, nxt[c], err);
         comp[i] = nc;
         bwn[i] = ~0ULL;
         ben[i] = ~0u;
@@ -1402,6 +1418,7 @@ static int spill_heights(lemhip_ctx *c, int nlab, int ne)
     const uint32_t *lo = c->elo, *hi = c->ehi;
     const unsigned long long *w = c->ew;
     int na = ne, buf = 0;
+    LEM_TRY(clear_flag(c));   // raised only by a pointer walk that does not end
     // the profile slot times the kernels only, not the host reads between them
     for (int ph = 0; na > 0; ph++) {
         const int p = ph & 1;
@@ -1411,30 +1428,23 @@ static int spill_heights(lemhip_ctx *c, int nlab, int ne)
         k_bv_min_e<<<ge, 256, 0, c->s>>>(na, lo, hi, w, c->comp, bw[p], be[p]);
         k_bv_hook<<<grid1d(n), 256, 0, c->s>>>(n, lo, hi, c->comp, be[p], nxt);
         k_bv_break<<<grid1d(n), 256, 0, c->s>>>(n, be[p], nxt);
-        prof_end(c, PROF_MM_K, 4);
+        // five jumping passes leave chains at most 1/32 as long; the root and
+        // label kernels walk the rest (no host read per phase for the jumps)
+        for (int q = 0; q < 5; q++) k_bv_jump<<<grid1d(n), 256, 0, c->s>>>(n, nxt);
+        prof_end(c, PROF_MM_K, 9);
         LEM_LAUNCHED(c);
-        for (int it = 0;; it++) {
-            LEM_TRY(clear_flag(c));
-            prof_begin(c, PROF_MM_K);
-            for (int q = 0; q < 4; q++) k_bv_jump<<<grid1d(n), 256, 0, c->s>>>(n, nxt, &c->scal->flag);
-            prof_end(c, PROF_MM_K, 4);
-            LEM_LAUNCHED(c);
-            int f = 0;
-            LEM_TRY(read_flag(c, &f));
-            if (!f) break;
-            if (it > 16) { c->err = "spill-height phases did not converge"; return LEMHIP_E_NOT_CONVERGED; }
-        }
         uint32_t *lo2 = c->elo2[buf], *hi2 = c->ehi2[buf];
         unsigned long long *w2 = c->ew2[buf];
         LEM_CHECK(c, hipMemsetAsync(&c->scal->nedges, 0, sizeof(int), c->s));
         prof_begin(c, PROF_MM_K);
-        k_bv_root0<<<1, 1, 0, c->s>>>(c->comp, bw[p], be[p], nxt, st);
+        k_bv_root0<<<1, 1, 0, c->s>>>(c->comp, bw[p], be[p], nxt, st, &c->scal->flag);
         k_bv_labels<<<grid1d(n), 256, 0, c->s>>>(n, c->comp, bw[p], be[p], nxt, mv, c->rmm, st, bw[p ^ 1],
-                                                  be[p ^ 1]);
+                                                  be[p ^ 1], &c->scal->flag);
         k_bv_keep<<<grid_chunks(na), 256, 0, c->s>>>(na, lo, hi, w, c->comp, lo2, hi2, w2, &c->scal->nedges);
         prof_end(c, PROF_MM_K, 3);
         LEM_LAUNCHED(c);
         LEM_TRY(read_scalars(c));
+        if (c->hscal->flag) { c->err = "spill-height component pointers form a cycle"; return LEMHIP_E_NOT_CONVERGED; }
         const int nk = c->hscal->nedges;
         // every component with an outgoing edge joins another each phase, so
         // the components with edges at least halve
