@@ -1531,8 +1531,9 @@ static int flooding(lemhip_ctx *c, int ncand1)
         break;
     }
     // edges: block-distinct pairs appended, sorted by pair, reduced to the
-    // minimum weight per pair; the list comes out sorted by label, so the
-    // relaxation passes read neighbouring labels' values from the same lines
+    // minimum weight per pair; the list comes out sorted by label (the
+    // Borůvka phases read neighbouring labels' components from the same
+    // lines, and their ties are broken by position in this order)
     LEM_TRY(ensure_labels(c, (long)nlab + 1));
     int B = 1;
     while ((1L << B) <= (long)nlab) B++;
@@ -1624,6 +1625,10 @@ static int tune_plateau(lemhip_ctx *c, const double *zs, TS *slopes)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // 8 rounds per launch fit the tile's region best (one region row per
+    // wave, five cells per thread): another count only when clearly faster,
+    // since contexts tuned side by side time each other's traffic too
+    if (bestk != 8 && best > 0.9f * (float)c->tune_ms[9]) bestk = 8;
     c->plat_k = bestk;
     c->tune_plat_pending = false;
     return LEMHIP_OK;
