@@ -57,17 +57,19 @@ __global__ void k_runoff(long N, const uint8_t *__restrict__ mask, const T *__re
 {
     bool b = false;
     for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
-        if (mask[self] == 0) {
+        const uint8_t m = mask[self];   // every input in flight before the mask test
+        const T st0 = stone[self], tb0 = TotB[self], sm0 = soilM[self], rn = rain[self];
+        if (m == 0) {
             w[self] = (T)-9999;
             fa_set(&fa[self], (T)0);
             continue;
         }
-        T runoff_coeff = ((T)1.0 - (T)0.0085 * (stone[self])) * mexp((T)-0.0025 * (TotB[self]) / (T)76.5)
-                         * mtanh((T)0.6 + (soilM[self]) / (T)0.2);
-        T myppt = rain[self] / (T)1000;
+        T runoff_coeff = ((T)1.0 - (T)0.0085 * (st0)) * mexp((T)-0.0025 * (tb0) / (T)76.5)
+                         * mtanh((T)0.6 + (sm0) / (T)0.2);
+        T myppt = rn / (T)1000;
         T wv = myppt * runoff_coeff;
         w[self] = wv;
-        T sm = soilM[self];
+        T sm = sm0;
         sm += myppt * ((T)1. - runoff_coeff);
         if (sm > (T)0.5) sm = (T)0.5;
         soilM[self] = sm;
@@ -92,7 +94,15 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
 {
     const double rescale_c = 1 / (10000 / (p.cellsize * p.cellsize));
     for (long self = blockIdx.x * (long)blockDim.x + threadIdx.x; self < N; self += (long)gridDim.x * blockDim.x) {
-        if (mask[self] == 0) {
+        // every input of the cell in flight before the mask test
+        const uint8_t m = mask[self];
+        T s8[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s8[j] = slopes[j * plane + self];
+        const double sTd = soilT[self];
+        const typename FaS<T>::type fav = fa[self];
+        const T tb0 = TotB[self], fi0 = fines[self], nu0 = nut[self], st0 = stone[self], tc = temp[self];
+        if (m == 0) {
             slopeptr[self] = 0; ero[self] = 0; inci[self] = 0; geli[self] = 0; depo[self] = 0;
             continue;
         }
@@ -100,18 +110,18 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         int nonzero = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            T sj = slopes[j * plane + self];
+            T sj = s8[j];
             total += sj;
             if (sj > 0) nonzero++;
         }
         T sp = 0;
         if ((total > 0) && (nonzero > 0)) sp = total / (T)nonzero;
         slopeptr[self] = sp;
-        T sT = (T)soilT[self];
-        const double fad = fa_full(fa[self]);
-        T ex = mexp((T)-0.07 * (TotB[self]) / (T)76.5);
+        T sT = (T)sTd;
+        const double fad = fa_full(fav);
+        T ex = mexp((T)-0.07 * (tb0) / (T)76.5);
         // diffuse
-        T erodibility = ((T)2.77e-6 * mpow(fines[self], (T)1.14) * ((T)12. - nut[self]))
+        T erodibility = ((T)2.77e-6 * mpow(fi0, (T)1.14) * ((T)12. - nu0))
                         + ((T)0.043 * ((T)p.soil_struct - (T)2.)) + ((T)0.033 * ((T)p.profile_permeability - (T)3.));
         if (erodibility < (T)0.0) erodibility = 0;
         double OFd = ((fad) * 1000) / 25;
@@ -121,7 +131,7 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         if (slope == (T)0.0) slope = (T)0.0001;
         if (slope > (T)5.0) slope = (T)5.0;
         T answer = ((erodibility * OF * OF * mpow(slope, (T)1.67) *
-                     (mpow(((T)110. - (stone[self])), (T)0.877) / (T)61.7023))) * ex;
+                     (mpow(((T)110. - (st0)), (T)0.877) / (T)61.7023))) * ex;
         if (answer > sT) answer = sT;
         ero[self] = answer;
         // concentrated
@@ -150,7 +160,6 @@ __global__ void __launch_bounds__(256) k_mod1(long N, long plane, lem_params p, 
         inci[self] = iv;
         // gelifluction
         T kappa = (T)p.kappa;
-        T tc = temp[self];
         if (tc >= (T)-20. && tc <= (T)8.87)
             kappa = (T)0.15711 - (T)0.011835 * tc - (T)0.001195 * tc * tc + (T)0.000038 * tc * tc * tc
                     + (T)0.000002 * tc * tc * tc * tc;
