@@ -76,7 +76,7 @@ typedef struct {
     int phantom_cells;
     int soilT_negative;
     /* FP32 form with lem_tuning.fp32_tree_sediment: the cells of the grid's
-     * last sediment pass left open by the engine's first round and routed by
+     * last sediment pass left open by the engine's first rounds and routed by
      * the contraction (0 when the engine ran to the end) */
     int sed_open;
 } lem_summary;
@@ -206,7 +206,7 @@ int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const in
  * accumulation does not use the engine; [0..5] and [11..13] then time
  * sediment routing, the pass the tile shape serves (its in-place fields are
  * saved and restored around the timed runs), unless fp32_tree_sediment is
- * on (its first round has a fixed tile shape; nothing is timed). */
+ * on (its engine rounds have a fixed tile shape; nothing is timed). */
 typedef struct {
     int dag_tile_w, dag_tile_h;
     int dag_batch;
@@ -231,7 +231,7 @@ typedef struct {
      * as the FP64 form does.  The same cells compute, every
      * field but stone is bit-identical between the two, and stone values
      * differ by a few float ulps where a composed chain reached them.  The
-     * tile shape of the first round is fixed (atlas rectangles are placed at
+     * tile shape of those rounds is fixed (atlas rectangles are placed at
      * multiples of 32, so a DEM's results do not depend on its rectangle) */
     int fp32_tree_sediment;
 } lem_tuning;
@@ -248,7 +248,7 @@ int lemhip_autotune(lemhip_ctx *ctx);
  * 11 flow-accumulation round kernels only, 12 sediment round kernels only,
  * 13 plateau round kernels only, 14 spill-height kernels only (the Borůvka
  * phases, without the host reads between them), 15 sediment contraction
- * round kernels only (slot 12 then holds the engine's first round).
+ * round kernels only (slot 12 then holds the engine's first rounds).
  * enable resets the accumulators; read returns milliseconds and launch
  * counts per slot. */
 int lemhip_profile_enable(lemhip_ctx *ctx, int on);

@@ -129,7 +129,7 @@ __global__ void k_slots_first_iter(int ns, const int *flags, DemScal *ds)
 // ---------------------------------------------------------- atlas packing
 
 // Every DEM starts at a column and a row that are multiples of LEM_ALIGN:
-// the fixed 32 x 32 tiling of the FP32 sediment pass's first round then
+// the fixed 32 x 32 tiling of the FP32 sediment pass's first rounds then
 // lies in the same place relative to each DEM as in a single-DEM context.
 // A DEM's rectangle with its nodata column takes its width rounded up to a
 // multiple of LEM_ALIGN (less at the grid's right edge), a shelf its height

@@ -1,8 +1,9 @@
 // lem_sed32.hip — sediment routing of the FP32 form under single flow
 // directions (MFDsedbud, mfd_accum.cu:63-252, loop mfd_accum.cu:345-483):
-// the dependency engine's first round over every tile, then a contraction of
-// what is left of the donor forest in synchronous rounds, instead of the
-// engine's further tile rounds.  (Under multiple flow directions most cells
+// the dependency engine's first rounds (the first over every tile), then a
+// contraction of what is left of the donor forest in synchronous rounds,
+// instead of the engine's further tile rounds.  (Under multiple flow
+// directions most cells
 // have several open donors at once, so few chains form and the rounds
 // approach the depth of the graph; those contexts keep the engine.)
 //
@@ -20,9 +21,11 @@
 //
 // The engine's first round visits every 32 x 32 tile once and completes the
 // cells whose whole upstream lies inside their tile (about 80% of the cells
-// on the fBm workloads); the later tile rounds would revisit most tiles
-// several times for the rest, and a river crossing hundreds of tiles needs
-// as many rounds.  The rest contracts instead:
+// on the fBm workloads), each later round the cells whose upstream enters
+// their tile only from cells completed before (three rounds by default);
+// further tile rounds would revisit most tiles for the rest, and a river
+// crossing hundreds of tiles needs as many rounds.  The rest contracts
+// instead:
 // functions x -> clamp(A + B x, L, U) are closed under composition, so the
 // graph contracts in rounds.  Each cell's state after round r is a function
 // of the states after round r-1 only (status words carry the round of their
@@ -40,7 +43,7 @@
 // A pointer that has passed more cells than its DEM holds went round a
 // cycle: the cell dies (never computes).  Rounds end when no cell is open.
 //
-// Values of the first round and of rake are bit-identical with the
+// Values of the engine's rounds and of rake are bit-identical with the
 // engine's.  A composed function is evaluated in double from the float
 // parameters, so values reached through one differ from the engine's
 // link-by-link float rounding by a few float ulps.  Which cells the first
@@ -167,7 +170,7 @@ __device__ __forceinline__ void cs_chunk_flush(CsChunk<IT> &C, int *list, int *c
     __syncthreads();
 }
 
-// Status words after the engine's first round (lem_sed_first): a cell it
+// Status words after the engine's first rounds (lem_sed_first): a cell they
 // completed is done and one it killed (or the prep kernel marked dead) is
 // dead, both with stamp 0; a pending cell gets its rake parameters, as the
 // engine's cell body forms them from its unchanged inputs, and is open.
@@ -380,7 +383,7 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
     // state has changed yet)
     if (c->hint[25]) return LEMHIP_OK;
     *fallback = 0;
-    // round 1: the engine's first round over every tile (its control block
+    // the engine's first rounds, the first over every tile (their control block
     // shares the memory of this one)
     LEM_TRY(lem_sed_first(c, c->cs_rounds));
     const long CH = 256L * CS_ITEMS;
