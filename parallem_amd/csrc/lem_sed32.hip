@@ -3,9 +3,9 @@
 // the dependency engine's first rounds (the first over every tile), then a
 // contraction of what is left of the donor forest in synchronous rounds,
 // instead of the engine's further tile rounds.  (Under multiple flow
-// directions most cells
-// have several open donors at once, so few chains form and the rounds
-// approach the depth of the graph; those contexts keep the engine.)
+// directions most cells have several open donors at once, so few chains
+// form and the rounds approach the depth of the graph; those contexts keep
+// the engine.)
 //
 // Of a computed cell's results only the stone fraction travels downstream:
 // deposition comes from the donors' erosion, incision and gelifluction
