@@ -1280,7 +1280,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     if (oe != hipSuccess || bpc < 1) bpc = 1;
     int G = bpc * c->ncu;
     if (G > ntiles) G = ntiles;
-    const int BATCH = 8;
+    const int BATCH = 16;   // launches between host checks (an empty launch costs less than a read)
     int r = 0;  // rounds done in this pass
     long guard = 0;
     for (;;) {

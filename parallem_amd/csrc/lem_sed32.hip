@@ -405,7 +405,7 @@ int lem_sed32(lemhip_ctx *c, int *fallback)
     c->h_sed_open = (int)open;
     uint32_t r = 1;
     long launches = 1;
-    const int BATCH = 4;
+    const int BATCH = 8;   // rounds between host reads of the open count
     while (open > 0) {
         prof_begin(c, PROF_CS_K);
         for (int q = 0; q < BATCH; q++) {
