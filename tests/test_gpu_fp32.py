@@ -333,3 +333,26 @@ def test_integer_accumulation_outlet_cycle():
         assert float(np.max(np.abs(b[m] - a[m]) / a[m])) <= 1e-6
         o = orc.s.outlet
         assert b[o] == np.float32(ctx.download("runoff")[o])      # cycle cells keep fa = w
+
+
+def test_tree_sediment_falls_back_on_non_finite_terms():
+    """The literal soil-thickness formula makes incision NaN below 400 m, so
+    deposition and the stone-mixing weights of cells downstream are not
+    finite: the sediment pass of the FP32 form then runs the engine's
+    rounds to the end (nothing of the state changes before the check), and
+    the results equal a context that always does, bit for bit."""
+    dem = synth.fbm_512(side=96, seed_offset=3)
+    H, W = dem.z.shape
+    out = []
+    for tree in (3, 0):
+        c = lemhip.LemContext(lemhip.default_params(W, H, 0), lemhip.FP32_KAHAN, 0)
+        c.init_state(dem.z, dem.rain, soilT_mode="formula")
+        c.set_tuning(fp32_tree_sediment=tree)
+        for k in range(3):
+            c.step(-1000 + k)
+            assert c.summary()["sed_open"] == 0, f"step {k}: the contraction ran on non-finite terms"
+        out.append({f: c.download(f) for f in ("dem", "stone", "fines", "soilT", "depo", "inci")})
+        c.close()
+    assert np.isnan(out[0]["inci"]).any()
+    for f in out[0]:
+        assert np.array_equal(out[0][f], out[1][f], equal_nan=True), f
