@@ -325,9 +325,10 @@ __global__ void __launch_bounds__(256) k_flowdirs(int W, int H, int cell_size, c
 // and recomputing the halo redundantly; after round k only cells at least k
 // cells from the halo's outer edge are current, so after K rounds the tile
 // itself is exact.  A cell can change in round r only if an equal-height
-// neighbour changed in round r-1, so a tile needs the next launch only when it
-// or one of its 8 neighbour tiles changed in the last round of this launch
-// (K <= tile width and height); tiles without candidates never change.
+// neighbour changed in round r-1, so a tile needs the next launch only when a
+// cell within K cells of it changed in the last round of this launch (K <=
+// tile width and height: in it or in one of its 8 neighbour tiles); tiles
+// without candidates never change.
 
 __global__ void k_sp_init(long N, float *sp)
 {
@@ -343,6 +344,38 @@ __global__ void k_sp_seed(int W, const int *cand, int n, float *sp, int ntx, int
     sp[i] = 20000000.0f;
     int t = ((i / W) / PTH) * ntx + (i % W) / PTW;
     if (tile_has[t] == 0) tile_has[t] = 1;
+}
+
+// A candidate with an equal-height neighbour at distance 0 (not a
+// candidate, or off the grid, where the plateau rounds read 0) takes its
+// distance in the first round; every other cell changes in round r only if
+// an equal-height neighbour changed in round r-1, i.e. it lies within r-1
+// cells of such a seed.  The first launch's K rounds (K <= tile height) can
+// therefore change only tiles in the 3x3 tile neighbourhood of a seed's
+// tile: those tiles (with candidates) are marked 3, the first list.
+__global__ void k_sp_seedmark(int W, int H, const int *cand, int n, const float *sp, const uint8_t *eqb, int ntx,
+                              int nty, int *tile_has)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = cand[k];
+    const int y = i / W, x = i % W;
+    const unsigned e = eqb[i];
+    bool seed = false;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        const int yy = y + cdy(d), xx = x + cdx(d);
+        if ((e >> d) & 1u) seed |= yy < 0 || yy >= H || xx < 0 || xx >= W || sp[(long)yy * W + xx] == 0.0f;
+    }
+    if (!seed) return;
+    const int ty = y / PTH, tx = x / PTW;
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            const int t2y = ty + dy, t2x = tx + dx;
+            if (t2y < 0 || t2y >= nty || t2x < 0 || t2x >= ntx) continue;
+            const int t2 = t2y * ntx + t2x;
+            if (tile_has[t2] != 0) tile_has[t2] = 3;
+        }
 }
 
 // Launch bookkeeping: launch L visits list L % 3 (count cnt[L % 3]); a tile
@@ -363,7 +396,7 @@ __global__ void k_plat_first_list(int ntiles, const int *tile_has, int *list, Pl
     __shared__ int s_base;
     for (int t0 = blockIdx.x * blockDim.x; t0 < ntiles; t0 += gridDim.x * blockDim.x) {
         int t = t0 + threadIdx.x;
-        bool have = t < ntiles && tile_has[t];
+        bool have = t < ntiles && tile_has[t] == 3;
         int at = block_append(&ctl->cnt[slot], have, s_wave, &s_base);
         if (have) list[at] = t;
     }
@@ -425,6 +458,7 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
     __shared__ uint8_t E[RN];
     __shared__ uint8_t code[NI], marks[NI];
     __shared__ int last;
+    __shared__ int s_qmask;   // 3x3 tiles within K cells of a cell that changed in round K
     __shared__ int s_buf[9];
     __shared__ int qbuf[PLAT_QCAP];   // tiles queued for the next launch, not yet appended
     __shared__ int qn;
@@ -442,7 +476,7 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
         tile_next = q0 + (int)gridDim.x < n ? act[q0 + gridDim.x] : 0;
         const int tx0 = (tile % ntx) * PTW, ty0 = (tile / ntx) * PTH;
         __syncthreads();
-        if (threadIdx.x == 0) last = 0;
+        if (threadIdx.x == 0) { last = 0; s_qmask = 0; }
         // load the region from both distance buffers, and in the same wait
         // which buffer holds each of the 3x3 tiles as of the last launch: the
         // region's loads do not wait for that choice (a visit of the first
@@ -496,6 +530,7 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
         __syncthreads();
         float *cur = A, *nxt = B;
         int moved = 0;   // last round in which one of this thread's tile cells changed
+        unsigned qm = 0;  // 3x3 tiles within K cells of this thread's cells that changed in round K
         // A round in which no cell of its window changes leaves every later
         // round unchanged too (round k+1 reads only round k values of a window
         // that contains its own), so the launch stops there.
@@ -528,6 +563,14 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
                     code[li] = (uint8_t)cd;
                     if (MARKS) marks[li] |= (uint8_t)mk;
                     moved = k;
+                    if (k == K) {
+                        const int ly = rr - K, lx = cc - K;
+                        const unsigned rows = 2u | (ly < K ? 1u : 0u) | (ly >= PTH - K ? 4u : 0u);
+                        const unsigned cols = 2u | (lx < K ? 1u : 0u) | (lx >= PTW - K ? 4u : 0u);
+#pragma unroll
+                        for (int ny = 0; ny < 3; ny++)
+                            if ((rows >> ny) & 1u) qm |= cols << (3 * ny);
+                    }
                 }
             }
             const int any = __syncthreads_or(changed);
@@ -535,13 +578,16 @@ __global__ void __launch_bounds__(PLAT_NT) k_plat_block(int W, int H, int launch
             if (!any) break;
         }
         if (moved) atomicMax(&last, moved);
+        if (qm) atomicOr(&s_qmask, (int)qm);
         __syncthreads();
-        // still moving at the end of the launch: this tile and its neighbours
-        // go on, gathered in LDS and appended to the next list with one
+        // still moving at the end of the launch: this tile and the neighbours
+        // within K cells of a cell that changed in the last round go on (a
+        // cell changing in round K + j lies within j cells of one that changed
+        // in round K), gathered in LDS and appended to the next list with one
         // atomic per full buffer (workgroups adding to one counter serialise).
         // Before the write-back: the wait for the atomic's returned value
         // would otherwise also wait for those stores
-        if (last == K && threadIdx.x < 9) {
+        if (threadIdx.x < 9 && ((s_qmask >> threadIdx.x) & 1)) {
             int ty = tile / ntx + (int)threadIdx.x / 3 - 1, tx = tile % ntx + (int)threadIdx.x % 3 - 1;
             if (ty >= 0 && ty < nty && tx >= 0 && tx < ntx) {
                 const int t2 = ty * ntx + tx;
@@ -1268,6 +1314,7 @@ static int directions_and_plateaus(lemhip_ctx *c, const double *zs, TS *slopes, 
     k_sp_init<<<grid1d(N), 256, 0, c->s>>>(N, sp_in);
     k_plat_tver_init<<<(ntiles + 255) / 256, 256, 0, c->s>>>(ntiles, tver);
     k_sp_seed<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, c->cand, n, sp_in, ntx, tile_has);
+    k_sp_seedmark<<<grid1d(n, 256, 1 << 30), 256, 0, c->s>>>(W, H, c->cand, n, sp_in, eqb, ntx, nty, tile_has);
     k_plat_first_list<<<grid1d(ntiles, 256, 1024), 256, 0, c->s>>>(ntiles, tile_has, L[launch % 3], ctl, launch % 3);
     LEM_LAUNCHED(c);
     int bpc = 0;
