@@ -254,3 +254,49 @@ def test_long_chain_of_basins(flowtype):
         _assert_fields(ctx, orc, f"step {k}", fields=["dem", "fa", "soilT", "stone", "depo"])
         _assert_summary(ctx, orc, f"step {k}")
     assert ctx.summary()["nlabels"] > 90
+
+
+def _wide_flats():
+    """A 200 x 262 surface falling west (and gently toward its middle row)
+    with a flat mesa (101 x 181 cells at 400 m) behind a 450 m wall, drained
+    through one notch on its west side, and a basin (51 x 221 cells,
+    200-210 m) behind a 260 m rim that spills east over one 230 m sill.
+    (A plane falling toward a corner of the frame would make the MFD
+    incision NaN next to the frame, in the oracle and here alike.)"""
+    H, W = 200, 262
+    y = np.arange(H)[:, None].astype(np.float64)
+    x = np.arange(W)[None, :].astype(np.float64)
+    z = 100.0 + 0.5 * x + 0.01 * np.abs(y - 100.0)
+    z[19:122, 59:242] = 450.0
+    z[20:121, 60:241] = 400.0
+    z[70, 59] = 400.0
+    rng = np.random.Generator(np.random.PCG64(7))
+    z[134:187, 29:252] = 260.0
+    z[135:186, 30:251] = 200.0 + 10.0 * rng.random((51, 221))
+    z[160, 251] = 230.0
+    z[0, :] = z[-1, :] = -9999.0
+    z[:, 0] = z[:, -1] = -9999.0
+    return synth.Dem(z=z, rain=np.full(z.shape, 1000.0), seed=0)
+
+
+@pytest.mark.parametrize("flowtype", [0, 1])
+def test_flats_drained_through_one_cell(flowtype):
+    """Plateau fronts that start at a single exit cell and cross many
+    plateau tiles in every direction: the mesa on the dem (181 rounds in the
+    first step) and the flooded basin on the raised surface (221 rounds), so
+    the first launch holds only the tiles next to the notch and the sill, and
+    each later launch only the tiles within reach of the last round's front.
+    Directions, slope marks, round counts and everything downstream match the
+    oracle bit for bit, at every rounds-per-launch choice."""
+    dem = _wide_flats()
+    for k_launch in (4, 8, 16):
+        ctx, orc = _pair(dem, flowtype)
+        ctx.set_tuning(plateau_rounds_per_launch=k_launch)
+        for k in range(2):
+            ctx.step(-1000 + k)
+            orc.step(-1000 + k)
+            _assert_fields(ctx, orc, f"K={k_launch} step {k}", fields=["dem", "fa", "soilT", "stone", "depo"])
+            _assert_summary(ctx, orc, f"K={k_launch} step {k}")
+        s = ctx.summary()
+        assert s["plateau_rounds2"] > 200
+        ctx.close()
