@@ -198,8 +198,9 @@ int lemhip_step_group(lemhip_ctx *const *ctxs, int n, const int *iters, const in
  * Results never depend on these choices.  lemhip_set_tuning cancels a
  * pending autotune.  lemhip_autotune makes the next step
  * time every candidate on the context's own data (the timed passes are
- * idempotent), keep the fastest (for the plateau rounds: 8 per launch unless
- * another count is at least 10% faster), and record the times in tuned_ms:
+ * idempotent), keep the fastest (for the plateau rounds: each count timed
+ * twice, keeping its faster time, and 8 per launch unless another count is
+ * at least 10% faster), and record the times in tuned_ms:
  * [0..5] flow accumulation per tile shape, [8..10] plateau routing per rounds
  * per launch, [11..13] flow accumulation with batches 4, 8, 32.  In the FP32
  * form under single flow directions with fp32_integer_fa (below), flow

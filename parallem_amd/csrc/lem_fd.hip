@@ -1653,12 +1653,16 @@ template <typename TS>
 static int tune_plateau(lemhip_ctx *c, const double *zs, TS *slopes)
 {
     static const int KS[3] = {4, 8, 16};
+    // each choice timed twice, in the order 8 4 16 16 4 8, keeping its
+    // faster time: a context tuned beside others sees their phases change
+    // between its timings, and one timing per choice picked 4 on some boxes
+    static const int ORDER[6] = {1, 0, 2, 2, 0, 1};
     hipEvent_t e0, e1;
     LEM_CHECK(c, hipEventCreate(&e0));
     LEM_CHECK(c, hipEventCreate(&e1));
-    float best = 1e30f;
-    int bestk = c->plat_k;
-    for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < 3; q++) c->tune_ms[8 + q] = 1e30;
+    for (int o = 0; o < 6; o++) {
+        const int q = ORDER[o];
         c->plat_k = KS[q];
         int n = 0, r = 0;
         LEM_CHECK(c, hipEventRecord(e0, c->s));
@@ -1667,11 +1671,14 @@ static int tune_plateau(lemhip_ctx *c, const double *zs, TS *slopes)
         LEM_CHECK(c, hipEventSynchronize(e1));
         float ms = 0.f;
         LEM_CHECK(c, hipEventElapsedTime(&ms, e0, e1));
-        c->tune_ms[8 + q] = ms;
-        if (ms < best) { best = ms; bestk = KS[q]; }
+        if (ms < c->tune_ms[8 + q]) c->tune_ms[8 + q] = ms;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    float best = 1e30f;
+    int bestk = 8;
+    for (int q = 0; q < 3; q++)
+        if ((float)c->tune_ms[8 + q] < best) { best = (float)c->tune_ms[8 + q]; bestk = KS[q]; }
     // 8 rounds per launch fit the tile's region best (one region row per
     // wave, five cells per thread): another count only when clearly faster,
     // since contexts tuned side by side time each other's traffic too
