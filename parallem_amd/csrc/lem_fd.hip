@@ -1008,19 +1008,42 @@ __global__ void k_bv_init(long n, uint32_t *comp, unsigned long long *mv, unsign
     }
 }
 
-// the lightest outgoing weight of each component
-__global__ void k_bv_min_w(int ne, const uint32_t *__restrict__ lo, const uint32_t *__restrict__ hi,
-                           const unsigned long long *__restrict__ w, const uint32_t *__restrict__ comp,
-                           unsigned long long *bw)
+// the lightest outgoing weight of each component.  The edge list is sorted
+// by (lo, hi), so the lanes of a wave whose edges leave one component on
+// the lo side sit in runs: each run's weights are reduced in the wave
+// (a minimum over edges of one component, in any grouping) and only the
+// run's first lane updates the component, instead of one atomic per lane
+// on the same address.
+__global__ void __launch_bounds__(256) k_bv_min_w(int ne, const uint32_t *__restrict__ lo,
+                                                  const uint32_t *__restrict__ hi,
+                                                  const unsigned long long *__restrict__ w,
+                                                  const uint32_t *__restrict__ comp, unsigned long long *bw)
 {
     const unsigned long long INF = ord_enc(lm_inf());
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ne) return;
-    const unsigned long long wk = w[k];
-    const uint32_t ca = comp[lo[k]], cb = comp[hi[k]];
-    if (ca == cb || wk >= INF) return;
-    if (wk < bw[ca]) atomicMin(&bw[ca], wk);
-    if (wk < bw[cb]) atomicMin(&bw[cb], wk);
+    const int lane = threadIdx.x & 63;
+    bool live = false;
+    unsigned long long wk = ~0ULL;
+    uint32_t ca = ~0u, cb = ~0u;
+    if (k < ne) {
+        wk = w[k];
+        ca = comp[lo[k]];
+        cb = comp[hi[k]];
+        live = ca != cb && wk < INF;
+    }
+    // lanes without an edge to count carry a key no component has
+    uint32_t key = live ? ca : ~0u;
+    unsigned long long v = live ? wk : ~0ULL;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long ov = __shfl_down(v, off, 64);
+        const uint32_t okey = __shfl_down(key, off, 64);
+        if (lane + off < 64 && okey == key && ov < v) v = ov;
+    }
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    if (live && head && v < bw[ca]) atomicMin(&bw[ca], v);
+    if (live && wk < bw[cb]) atomicMin(&bw[cb], wk);
 }
 
 // the first edge of that weight
