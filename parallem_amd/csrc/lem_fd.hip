@@ -1013,7 +1013,7 @@ __global__ void k_bv_init(long n, uint32_t *comp, unsigned long long *mv, unsign
 // the lo side sit in runs: each run's weights are reduced in the wave
 // (a minimum over edges of one component, in any grouping) and only the
 // run's first lane updates the component, instead of one atomic per lane
-// on the same address.
+// on the same address; the same for runs on the hi side.
 __global__ void __launch_bounds__(256) k_bv_min_w(int ne, const uint32_t *__restrict__ lo,
                                                   const uint32_t *__restrict__ hi,
                                                   const unsigned long long *__restrict__ w,
@@ -1031,19 +1031,24 @@ __global__ void __launch_bounds__(256) k_bv_min_w(int ne, const uint32_t *__rest
         cb = comp[hi[k]];
         live = ca != cb && wk < INF;
     }
-    // lanes without an edge to count carry a key no component has
-    uint32_t key = live ? ca : ~0u;
-    unsigned long long v = live ? wk : ~0ULL;
+    // lanes without an edge to count carry a key no component has; the hi
+    // side's components repeat in runs too once components have merged
+    uint32_t key = live ? ca : ~0u, key2 = live ? cb : ~0u;
+    unsigned long long v = live ? wk : ~0ULL, v2 = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const unsigned long long ov = __shfl_down(v, off, 64);
-        const uint32_t okey = __shfl_down(key, off, 64);
+        const unsigned long long ov = __shfl_down(v, off, 64), ov2 = __shfl_down(v2, off, 64);
+        const uint32_t okey = __shfl_down(key, off, 64), okey2 = __shfl_down(key2, off, 64);
         if (lane + off < 64 && okey == key && ov < v) v = ov;
+        if (lane + off < 64 && okey2 == key2 && ov2 < v2) v2 = ov2;
     }
-    const uint32_t prev = __shfl_up(key, 1, 64);
+    // every lane takes part in the shuffles (a shuffle under a branch would
+    // read the inactive lane 0 as 0, component 0's key)
+    const uint32_t prev = __shfl_up(key, 1, 64), prev2 = __shfl_up(key2, 1, 64);
     const bool head = lane == 0 || prev != key;
+    const bool head2 = lane == 0 || prev2 != key2;
     if (live && head && v < bw[ca]) atomicMin(&bw[ca], v);
-    if (live && wk < bw[cb]) atomicMin(&bw[cb], wk);
+    if (live && head2 && v2 < bw[cb]) atomicMin(&bw[cb], v2);
 }
 
 // the first edge of that weight
