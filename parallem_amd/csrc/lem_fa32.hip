@@ -313,19 +313,43 @@ __global__ void k_fx_gather(int n, const unsigned long long *val, const int *P, 
     }
 }
 
-__global__ void k_fx_scatter(int n, unsigned long long *val, int *P, const unsigned long long *tv, const int *tp,
-                             int *flag)
+// Late rounds point many exits at the same few roots: the lanes of a wave
+// with equal parents sit in runs, and each run's values are summed in the
+// wave (a sum over the run's own lanes only, so every value counts once)
+// and added by the run's first lane with one atomic.  Integer sums: the
+// totals do not depend on the grouping.
+__global__ void __launch_bounds__(256) k_fx_scatter(int n, unsigned long long *val, int *P,
+                                                    const unsigned long long *tv, const int *tp, int *flag)
 {
+    const int lane = threadIdx.x & 63;
     bool more = false;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int p = P[k];
-        if (p >= 0) {
-            atomicAdd(&val[p], tv[k]);
-            P[k] = tp[k];
-            more |= tp[k] >= 0;
+    // whole waves go round the loop together (the shuffles need every lane)
+    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {
+        const int k = k0 + (int)threadIdx.x;
+        int p = -1;
+        unsigned long long v = 0;
+        if (k < n) {
+            p = P[k];
+            if (p >= 0) {
+                v = tv[k];
+                const int q = tp[k];
+                P[k] = q;
+                more |= q >= 0;
+            }
         }
+        const int pprev = __shfl_up(p, 1, 64);
+        const bool head = lane == 0 || pprev != p;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long above = lane == 63 ? 0ULL : heads & (~0ULL << (lane + 1));
+        const int end = above ? __ffsll((long long)above) - 1 : 64;   // first lane of the next run
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned long long o = __shfl_down(v, off, 64);
+            if (lane + off < end) v += o;
+        }
+        if (head && p >= 0) atomicAdd(&val[p], v);
     }
-    if (__any(more) && (threadIdx.x & 63) == 0) *flag = 1;
+    if (__any(more) && lane == 0) *flag = 1;
 }
 
 // tagged cells: the cycle through the outlet (fa = w, never completed), or
